@@ -1,0 +1,160 @@
+"""Benchmark: zone-updates/s of the MHD step on the 256^3 shearing box.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+
+One "step" is one full time step of the shearing-box MRI configuration
+(ghost fill, CFL reduction, two RK stages) on a 256^3 box (array 256x264x256
+with 4 radial ghost layers per side), Omega = 1, q = 1.5, gamma = 5/3,
+beta = 400, seeded velocity noise.  The state stays resident in HBM; the
+timed region covers exactly K steps enqueued back to back, bracketed by
+device synchronisation.  value = interior cells * K / seconds.
+
+The JSON line also carries
+  roofline     -- the dominant kernel's algorithmic HBM bytes per launch over
+                  its mean launch time (HIP events on the library's stream),
+                  against the 8 TB/s HBM peak;
+  step_roofline -- the whole step: 160 algorithmic bytes per zone-update;
+  cpu_baseline -- the oracle (CPU restatement of the reference step) on the
+                  GPU box's host cores, a 2-step sample of the same box.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+STEP_BYTES = 160.0  # algorithmic bytes per zone-update (8 fields x 5 passes x 4 B)
+# Algorithmic bytes per interior cell of one launch of each kernel.
+KERNEL_BYTES = {
+    "stage1": 64.0,      # read W^n (8), write W* (8)
+    "stage2": 96.0,      # read W* (8) and W^n (8), write W^{n+1} (8)
+    "flux_x": 60.0,      # read W (8), write 7 face fluxes
+    "flux_y": 60.0,
+    "flux_z": 60.0,
+    "update_s1": 160.0,  # read 21 fluxes, 3 winds, 8 base fields, write 8
+    "update_s2": 160.0,
+    "cfl": 32.0,         # read 8
+}
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--size", type=int, nargs=3, default=(256, 256, 256))
+    ap.add_argument("--path", choices=("auto", "unfused", "fused"), default="auto")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--profile-steps", type=int, default=10)
+    return ap.parse_args()
+
+
+def cpu_baseline(cfg, W, steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle
+    threads = min(16, os.cpu_count() or 1)
+    orc = pyoracle.Oracle(cfg, W, threads=threads)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        orc.step()
+    el = time.perf_counter() - t0
+    orc.close()
+    from nmhd_amd import problems as P
+    return {
+        "value": P.interior_cells(cfg) * steps / el,
+        "unit": "zone-updates/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": "%d steps of the same %dx%dx%d shearing box on the oracle "
+                  "(C restatement of the reference step, OpenMP, %d threads)"
+                  % ((steps,) + tuple(cfg["global_size"]) + (threads,)),
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
+              "(the multi-GPU decomposition is described in DESIGN.md)" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from nmhd_amd import capi, problems as P
+
+    torch.cuda.set_device(0)
+    cfg = P.config_c2(tuple(a.size))
+    W = P.shearing_box_mri(cfg, seed=1002)
+    path = {"auto": capi.PATH_AUTO, "unfused": capi.PATH_UNFUSED,
+            "fused": capi.PATH_FUSED}[a.path]
+    ctx = capi.Context(cfg, path=path)
+    ctx.set_state(W)
+    goal = 1e30
+    zones = P.interior_cells(cfg)
+
+    ctx.reset_run()
+    ctx.enqueue_steps(goal, a.warmup)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.enqueue_steps(goal, a.steps)
+    done, _ = ctx.synchronize()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    assert done == a.warmup + a.steps, (done, a.warmup, a.steps)
+    value = zones * a.steps / el
+
+    # per-kernel times with HIP events on the library's stream
+    ctx.timing_enable(True)
+    ctx.timing_reset()
+    ctx.enqueue_steps(goal, a.profile_steps)
+    ctx.synchronize()
+    tim = ctx.timing()
+    ctx.timing_enable(False)
+    kern = {k: v for k, v in tim.items() if v[1] > 0 and k in KERNEL_BYTES}
+    dom = max(kern, key=lambda k: kern[k][0])
+    mean_s = kern[dom][0] / kern[dom][1] / 1e3
+    achieved = KERNEL_BYTES[dom] * zones / mean_s
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                "kernel": dom, "mean_launch_ms": mean_s * 1e3,
+                "algorithmic_bytes_per_launch": KERNEL_BYTES[dom] * zones}
+    step_achieved = value * STEP_BYTES
+    out = {
+        "metric": "zone-updates/s at 256^3 fp32, % HBM roofline; batched & adjoint vs plain",
+        "value": value,
+        "unit": "zone-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded shearing-box MRI initial state)",
+        "config": {"workload": "shearing-box MRI step, %dx%dx%d cells (array %dx%dx%d incl. "
+                               "4 radial ghost layers/side), gamma 5/3, beta 400, CFL 0.4"
+                               % (tuple(cfg["global_size"]) + tuple(cfg["n"])),
+                   "global_size": cfg["global_size"], "steps": a.steps,
+                   "path": ["auto", "unfused", "fused"][ctx.get_path()]},
+        "roofline": roofline,
+        "step_roofline": {"achieved": step_achieved / 1e9, "peak": HBM_PEAK / 1e9,
+                          "unit": "GB/s", "frac": step_achieved / HBM_PEAK,
+                          "bytes_per_zone_update": STEP_BYTES},
+        "kernel_ms_per_step": {k: v[0] / a.profile_steps for k, v in tim.items() if v[1]},
+    }
+    ctx.close()
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, W, a.cpu_steps)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,191 @@
+/*
+ * nmhd.h -- C ABI of the MI355X-native ideal-MHD step (libnmhd.so).
+ *
+ * The reference (nushio3/nmhd) has no FFI: its "operator API" is the C++
+ * class MHDSolver (library/mhd_solver.h:514-640, template
+ * library/mhd_solver.hrb:150-276) whose methods launch CUDA kernels.  This
+ * header is the thin C layer that the C++ facade (include/nmhd/mhd_solver.h)
+ * and any other host language call; each entry point names the reference
+ * interface it replaces.  Paths are relative to the reference's src/ tree;
+ * INL = library/mhd_solver.inl.
+ *
+ * Conventions
+ *  - Every function returns an int status: NMHD_OK (0) or an NMHD_E* code;
+ *    nmhd_last_error() gives the message of the last failure on a context.
+ *  - Field arrays are structure-of-arrays FP32, one array per field in the
+ *    order rho, vx, vy, vz, p, Bx, By, Bz (the order of the reference's .bin
+ *    files, INL:1226-1244).  Each array holds the whole box including ghost
+ *    layers: addr = (z*ny + y)*nx + x, x fastest (crystal.h:31-50).  Face
+ *    B_d[i] is stored at the lower d-face of cell i.
+ *  - A context owns its device buffers and one HIP stream.  Calls on one
+ *    context must be serialised by the caller; different contexts may be
+ *    used concurrently.  Host arrays passed in are read or written before
+ *    the call returns.
+ *  - Shearing box orientation is the reference's: radial = Y (ghost layers
+ *    along Y), azimuth = X, vertical = Z (application/kepler_disk_demo).
+ */
+#ifndef NMHD_H
+#define NMHD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum nmhd_status {
+  NMHD_OK = 0,
+  NMHD_EINVAL = 1,   /* bad argument or descriptor                 */
+  NMHD_EHIP = 2,     /* a HIP runtime call failed                  */
+  NMHD_ENOMEM = 3,   /* device or host allocation failed           */
+  NMHD_EIO = 4,      /* file could not be opened / sizes mismatch  */
+  NMHD_ESTATE = 5    /* call not valid in the context's state      */
+};
+
+enum nmhd_source {
+  NMHD_SOURCE_NONE = 0,
+  /* Coriolis + tidal acceleration of the shearing box, the hooks
+   * external_acceleration / relative_velocity / initial_displacement of
+   * application/kepler_disk_demo/mhd_solver.curb:46-81 */
+  NMHD_SOURCE_SHEARING_BOX = 1
+};
+
+enum nmhd_recon {
+  NMHD_RECON_MINBEE2 = 0,        /* minmod MUSCL, INL:68-84            */
+  NMHD_RECON_PIECEWISECONST1 = 1 /* first order, mhd_solver.inlrb:128-135 */
+};
+
+enum nmhd_integrator {
+  NMHD_MIDPOINT2 = 0, /* two-stage midpoint, mhd_solver.inlrb:904-978 */
+  NMHD_EULER1 = 1     /* forward Euler, mhd_solver.inlrb:966-978      */
+};
+
+enum nmhd_path {
+  NMHD_PATH_AUTO = 0,     /* fastest path valid for the descriptor        */
+  NMHD_PATH_UNFUSED = 1,  /* per-direction flux kernels + one update pass */
+  NMHD_PATH_FUSED = 2,    /* one fused stencil kernel per RK stage        */
+  /* verification path: the unfused kernels with the reference's rounding
+   * (no FMA contraction, IEEE division and square root, the FP64-promoted
+   * HLLD literals); bit-identical to the CPU restatement of the reference */
+  NMHD_PATH_STRICT = 3
+};
+
+/* Box description; the compile-time constants of the reference's
+ * simulation_config.h (simulation_config_default.h:25-122) as run-time
+ * values.  Derived constants (kExtent*, kD*, kLocalExtent*) are computed in
+ * FP32 exactly as the reference's constant expressions are. */
+typedef struct nmhd_desc {
+  int global_size[3];      /* interior cells per axis (kGlobalSizeX..)          */
+  int margin[3];           /* ghost layers per side (kMarginSizeX..); [0] == 0  */
+  float global_extent[3];  /* physical extent of the interior (kGlobalExtentX..) */
+  float gamma;             /* kGamma                                            */
+  float cfl_factor;        /* kCflFactor                                        */
+  int source;              /* enum nmhd_source                                  */
+  float omega;             /* kOmega (shearing box)                             */
+  float qshear;            /* kDifferentialRotationIndex (shearing box)         */
+  int shear_y;             /* Y ghost fill: 0 periodic copy, 1 shearing-periodic */
+  int recon;               /* enum nmhd_recon                                   */
+  int integrator;          /* enum nmhd_integrator                              */
+  int path;                /* enum nmhd_path                                    */
+} nmhd_desc;
+
+/* Derived geometry (LocalSetup members, mhd_solver.hrb:93-101). */
+typedef struct nmhd_geometry {
+  int size[3];            /* array size incl. ghosts (kSizeX..)   */
+  size_t ncell;           /* size[0]*size[1]*size[2] (kSize)      */
+  float extent[3];        /* array extent incl. ghosts (kExtentX..) */
+  float local_extent[3];  /* interior extent (kLocalExtentX..)    */
+  float d[3];             /* cell spacing (kDX..)                 */
+} nmhd_geometry;
+
+typedef struct nmhd_ctx nmhd_ctx;
+
+/* MHDSolver::MHDSolver (INL:14-41): allocate the device state for one box on
+ * the current HIP device. */
+int nmhd_create(const nmhd_desc *desc, nmhd_ctx **out);
+/* ~MHDSolver: free the context and its device memory. */
+int nmhd_destroy(nmhd_ctx *ctx);
+/* Message of the last failure on ctx (or of the last failed nmhd_create when
+ * ctx is NULL). */
+const char *nmhd_last_error(const nmhd_ctx *ctx);
+
+int nmhd_get_geometry(const nmhd_ctx *ctx, nmhd_geometry *geo);
+
+/* MHDSolver::read / write (INL:1207-1244) without the file: copy the eight
+ * state fields in from / out to host memory. */
+int nmhd_set_state(nmhd_ctx *ctx, const float *const fields[8]);
+int nmhd_get_state(nmhd_ctx *ctx, float *const fields[8]);
+/* Same with device pointers (ncell floats each), e.g. torch tensors. */
+int nmhd_set_state_device(nmhd_ctx *ctx, const float *const dev_fields[8]);
+int nmhd_get_state_device(nmhd_ctx *ctx, float *const dev_fields[8]);
+
+/* GlobalSetup::current_time / generation (mhd_solver.hrb:47-59). */
+int nmhd_set_time(nmhd_ctx *ctx, double t, int generation);
+int nmhd_get_time(nmhd_ctx *ctx, double *t, int *generation);
+
+/* MHDSolver::communicate (INL:307-711) for one rank: shearing-periodic (or
+ * periodic) Y ghost fill with relative_displacement (INL:288-295) at the
+ * context's current time, then the periodic Z ghost copy. */
+int nmhd_fill_ghosts(nmhd_ctx *ctx);
+
+/* relative_displacement (INL:288-295) at time t with the shearing-box hooks
+ * (application/kepler_disk_demo/mhd_solver.curb:46-52). */
+int nmhd_relative_displacement(const nmhd_ctx *ctx, double t, float *normalized,
+                               int *int_displacement, float *minor_displacement);
+
+/* cfl_condition_kernel + thrust::min_element (INL:714-739, 1106-1116): the
+ * minimum over every cell (ghosts included) of the CFL time, at most
+ * goal_time.  Synchronises. */
+int nmhd_cfl_dt(nmhd_ctx *ctx, float goal_time, float *dt);
+
+/* clear_pace_kernel + add_flux_kernel_caller (INL:741-755, 918-988): the
+ * right-hand side P(W) of the current state with stage step h (used by the
+ * face predictor 0.5*h*a), written to host arrays in the order rho, mx, my,
+ * mz, E, Bx, By, Bz.  Synchronises. */
+int nmhd_rhs(nmhd_ctx *ctx, float h, float *const pace[8]);
+
+/* One iteration of MHDSolver::proceed's loop (INL:1094-1201): ghost fill,
+ * CFL dt clipped to goal_time, the two RK stages, t += dt.  Synchronises and
+ * reports the dt used and whether the step reached goal_time. */
+int nmhd_step(nmhd_ctx *ctx, float goal_time, float *dt, int *goal_reached);
+
+/* MHDSolver::proceed (INL:1086-1204) without the monitor/wallclock hooks
+ * (the C++ facade adds them): run at most max_steps steps toward goal_time,
+ * enqueued in chunks with dt kept on the device.  Synchronises. */
+int nmhd_run(nmhd_ctx *ctx, float goal_time, int max_steps, int *steps_taken,
+             int *goal_reached);
+
+/* Asynchronous form: nmhd_reset_run starts a new run (clears the
+ * goal-reached latch and the step counter); nmhd_enqueue_steps enqueues
+ * nsteps iterations of the proceed loop without host round trips (dt is
+ * computed, clipped and consumed on the device; once a step reaches
+ * goal_time the remaining enqueued steps do nothing) and returns at once;
+ * nmhd_synchronize waits and reports the steps executed since the reset and
+ * whether goal_time was reached. */
+int nmhd_reset_run(nmhd_ctx *ctx);
+int nmhd_enqueue_steps(nmhd_ctx *ctx, float goal_time, int nsteps);
+int nmhd_synchronize(nmhd_ctx *ctx, int *steps_taken, int *goal_reached);
+
+/* The device HLLD solver (hlld.inl:7-284) on n host face states, with the
+ * fast arithmetic or, for path == NMHD_PATH_STRICT, the reference's rounding:
+ * in[15] = Bx, rhoL, rhoR, vxL, vxR, vyL, vyR, vzL, vzR, pL, pR, ByL, ByR,
+ * BzL, BzR; out[7] = F_rho, F_mx, F_my, F_mz, F_E, F_By, F_Bz. */
+int nmhd_hlld_batch(size_t n, float gamma, int path, const float *const in[15],
+                    float *const out[7]);
+
+/* Per-kernel timing with HIP events on the context's stream. */
+int nmhd_timing_enable(nmhd_ctx *ctx, int enable);
+int nmhd_timing_reset(nmhd_ctx *ctx);
+/* Kernel slot i: name, total milliseconds and launches since reset;
+ * NMHD_EINVAL past the last slot. */
+int nmhd_timing_get(nmhd_ctx *ctx, int i, const char **name, double *total_ms,
+                    long *launches);
+
+/* Select the kernel path (enum nmhd_path) for later steps. */
+int nmhd_set_path(nmhd_ctx *ctx, int path);
+int nmhd_get_path(const nmhd_ctx *ctx, int *path);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NMHD_H */

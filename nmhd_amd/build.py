@@ -1,0 +1,52 @@
+"""Build libnmhd.so (HIP, gfx950) in-tree with explicit hipcc commands.
+
+The library is the product: nmhd_amd/libnmhd.so, C ABI in include/nmhd.h.
+Every HIP translation unit is compiled with packed-FP32 instruction
+selection disabled.
+"""
+import os
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libnmhd.so")
+
+SOURCES = ["capi.hip", "kernels_common.hip", "kernels_unfused.hip", "kernels_strict.hip"]
+# kernels_strict.hip reproduces the reference's rounding: no FMA contraction.
+FILE_FLAGS = {"kernels_strict.hip": ["-ffp-contract=off"]}
+
+HIPFLAGS = [
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
+    "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build(jobs=8, extra=()):
+    os.makedirs(OBJ, exist_ok=True)
+    objs = []
+    cmds = []
+    for src in SOURCES:
+        obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+        objs.append(obj)
+        cmds.append(["hipcc", *HIPFLAGS, *FILE_FLAGS.get(src, []), *extra, "-c",
+                     os.path.join(CSRC, src), "-o", obj])
+    with ThreadPoolExecutor(max_workers=jobs) as ex:
+        list(ex.map(_run, cmds))
+    _run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", LIB])
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,248 @@
+// kernels_common.hip -- kernels shared by every path: step bookkeeping on the
+// device, the shearing-periodic ghost fill, the periodic Z ghost copy, the
+// CFL reduction and the HLLD diagnostic.
+//
+// Reference correspondence (INL = library/mhd_solver.inl):
+//   k_ghost_y   communicate_gather_kernel_y + communicate_scatter_kernel_y
+//               (INL:90-286) with relative_displacement (INL:288-295)
+//   k_ghost_z   the Z halo copies of communicate() (INL:446-708)
+//   k_cfl       cfl_condition_kernel (INL:714-739) + thrust::min_element
+//               (INL:1111-1116), as one pass with a wave/block/grid min
+//   k_dt        dt clipping of proceed() (INL:1126-1137)
+//   k_commit    GlobalSetup::proceed (mhd_solver.hrb:56-59)
+#include "nmhd_kernels.h"
+
+namespace nmhd {
+
+__global__ void k_begin_step(StepState *st) {
+  st->cfl_bits = 0x7f800000u;  // +inf
+}
+
+// relative_displacement (INL:288-295) with D0 = 0 and
+// relvel = -q*Omega*kLocalExtentY (application/kepler_disk_demo/
+// mhd_solver.curb:46-52), in FP64 as the reference evaluates it.
+__host__ __device__ inline void displacement(const Box &b, double t, float *normalized,
+                                             int *n, float *minor) {
+  const float relvel = -b.qshear * b.omega * b.local_ext[1];
+  const double d0 = 0.0 + t * (double)relvel;
+  const int fold = (int)floor(d0 / (double)b.ext[0]);
+  const double d1 = d0 - (double)((float)fold * b.ext[0]);
+  *normalized = (float)d1;
+  *n = (int)floor(d1 / (double)b.d[0]);
+  *minor = (float)(d1 / (double)b.d[0] - *n);
+}
+
+void host_displacement(const Box &b, double t, float *normalized, int *n, float *minor) {
+  displacement(b, t, normalized, n, minor);
+}
+
+// One thread per (x, row, z) of the M-row ghost slab: the lower ghost row j
+// takes the top interior row j shifted by +n along X, the upper ghost row
+// takes the bottom interior row shifted by -n-1, v_X offset by -/+ relvel;
+// the first upper ghost By face (the outer radial face) is kept.
+__global__ void k_ghost_y(Box b, FieldPtrs W, const StepState *st) {
+#pragma clang fp contract(off)
+  if (st->finished) return;
+  const int nx = b.n[0], ny = b.n[1], M = b.m[1];
+  const long unit = (long)nx * M * b.n[2];
+  const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= unit) return;
+  const int sx = (int)(a % nx);
+  const int sy = (int)((a / nx) % M);
+  const int sz = (int)(a / nx / M);
+  int n = 0;
+  float dr = 0.0f, nd, relvel = 0.0f;
+  int ix = sx, dx = sx;
+  if (b.shear_y) {
+    displacement(b, st->t, &nd, &n, &dr);
+    relvel = -b.qshear * b.omega * b.local_ext[1];
+    n %= nx;  // same shift, and no int overflow in sx + n for huge t
+    ix = ((sx + n) % nx + nx) % nx;
+    dx = ((sx - n - 1) % nx + nx) % nx;
+  }
+  const long src_inc = addr(b, ix, ny - 2 * M + sy, sz);
+  const long src_dec = addr(b, dx, M + sy, sz);
+  const long dst_inc = addr(b, sx, sy, sz);
+  const long dst_dec = addr(b, sx, ny - M + sy, sz);
+  for (int f = 0; f < 8; ++f) {
+    float vi = W.f[f][src_inc];
+    float vd = W.f[f][src_dec];
+    if (b.shear_y) {
+      // (1-d)*v + d*v, exactly as the gather kernel writes it (INL:108-110)
+      float bi = (1.0f - dr) * vi + dr * vi;
+      float bd = dr * vd + (1.0f - dr) * vd;
+      if (f == 1) {
+        bi = bi - relvel;
+        bd = bd + relvel;
+      }
+      vi = bi;
+      vd = bd;
+    }
+    W.f[f][dst_inc] = vi;
+    if (!(f == 6 && sy == 0)) W.f[f][dst_dec] = vd;
+  }
+}
+
+// Z ghost planes: [0, M) <- [nz-2M, nz-M), [nz-M, nz) <- [M, 2M).
+__global__ void k_ghost_z(Box b, FieldPtrs W, const StepState *st) {
+  if (st->finished) return;
+  const int M = b.m[2];
+  const long plane = (long)b.n[0] * b.n[1];
+  const long unit = plane * M;
+  const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= unit) return;
+  for (int f = 0; f < 8; ++f) {
+    float *p = W.f[f];
+    p[a] = p[b.ncell - 2 * unit + a];
+    p[b.ncell - unit + a] = p[unit + a];
+  }
+}
+
+__device__ __forceinline__ float wave_min(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// Per-cell CFL time min(goal, C*d/|v_d + sqrt((gamma p + b2)/rho)|) with
+// b2 = sum_d max(B_d,lo^2, B_d,hi^2), reduced to one float with a wave
+// shuffle min, an LDS block min and one atomic min per block on the float
+// bits (valid: all candidates are >= 0).
+__global__ void __launch_bounds__(256) k_cfl(Box b, FieldPtrs W, float goal, StepState *st) {
+#pragma clang fp contract(off)
+  if (st->finished) return;
+  __shared__ float red[4];
+  const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  float t = goal;
+  if (a < b.ncell) {
+    const int x = (int)(a % b.n[0]);
+    const int y = (int)((a / b.n[0]) % b.n[1]);
+    const int z = (int)(a / b.n[0] / b.n[1]);
+    const float bxl = W.f[5][a], bxh = W.f[5][addr(b, wrap(x + 1, b.n[0]), y, z)];
+    const float byl = W.f[6][a], byh = W.f[6][addr(b, x, wrap(y + 1, b.n[1]), z)];
+    const float bzl = W.f[7][a], bzh = W.f[7][addr(b, x, y, wrap(z + 1, b.n[2]))];
+    const float b2 = fmaxf(bxl * bxl, bxh * bxh) + fmaxf(byl * byl, byh * byh) +
+                     fmaxf(bzl * bzl, bzh * bzh);
+    const float c = sqrtf((b.gamma * W.f[4][a] + b2) / W.f[0][a]);
+    for (int d = 0; d < 3; ++d) {
+      const float speed = fabsf(W.f[1 + d][a] + c);
+      t = fminf(t, b.cfl * b.d[d] / speed);
+    }
+  }
+  t = wave_min(t);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = red[0];
+    for (int i = 1; i < (int)(blockDim.x >> 6); ++i) m = fminf(m, red[i]);
+    atomicMin(&st->cfl_bits, __float_as_uint(m));
+  }
+}
+
+// Decide the step: dt = CFL minimum, clipped so that t + dt <= goal.
+__global__ void k_dt(StepState *st, float goal) {
+  if (st->finished) {
+    st->active = 0;
+    return;
+  }
+  float dt = __uint_as_float(st->cfl_bits);
+  const double next = st->t + (double)dt;
+  if (next > (double)goal) {
+    dt = (float)((double)goal - st->t);
+    st->finished = 1;
+    st->goal_reached = 1;
+  }
+  st->dt = dt;
+  st->h = dt / 2;
+  st->t_next = st->t + (double)dt;
+  st->active = 1;
+}
+
+__global__ void k_commit(StepState *st) {
+  if (!st->active) return;
+  st->t = st->t_next;
+  st->generation += 1;
+  st->steps_done += 1;
+  st->active = 0;
+}
+
+__global__ void k_hlld_batch(long n, float gamma, FluxIn in, FluxOut out) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float inv_gm1 = (float)(1.0 / (double)(gamma - 1.0f));
+  Flux7 F = hlld(gamma, inv_gm1, in.f[0][i], in.f[1][i], in.f[2][i], in.f[3][i], in.f[4][i],
+                 in.f[5][i], in.f[6][i], in.f[7][i], in.f[8][i], in.f[9][i], in.f[10][i],
+                 in.f[11][i], in.f[12][i], in.f[13][i], in.f[14][i]);
+  out.f[0][i] = F.rho;
+  out.f[1][i] = F.mn;
+  out.f[2][i] = F.mt1;
+  out.f[3][i] = F.mt2;
+  out.f[4][i] = F.e;
+  out.f[5][i] = F.bt1;
+  out.f[6][i] = F.bt2;
+}
+
+// Copy the state (used when an enqueued step is inactive: the rotation of
+// the state buffers still has to carry the state forward).
+__global__ void k_copy_state(Box b, FieldPtrs src, FieldPtrs dst, const StepState *st) {
+  if (st->active) return;
+  const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= b.ncell) return;
+  for (int f = 0; f < 8; ++f) dst.f[f][a] = src.f[f][a];
+}
+
+// Diagnostic entry points run on their own step record: current time of the
+// run, active, not finished, explicit step h.
+__global__ void k_diag_prepare(StepState *diag, const StepState *st, float h) {
+  diag->t = st->t;
+  diag->t_next = st->t;
+  diag->dt = h;
+  diag->h = h;
+  diag->h_diag = h;
+  diag->cfl_bits = 0x7f800000u;
+  diag->finished = 0;
+  diag->active = 1;
+  diag->generation = st->generation;
+  diag->steps_done = 0;
+  diag->goal_reached = 0;
+}
+
+// ------------------------------------------------------------- launchers
+
+void launch_begin_step(hipStream_t s, StepState *st) {
+  hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, st);
+}
+void launch_ghost_y(hipStream_t s, const Box &b, FieldPtrs W, const StepState *st) {
+  const long unit = (long)b.n[0] * b.m[1] * b.n[2];
+  hipLaunchKernelGGL(k_ghost_y, dim3(grid_for(unit)), dim3(kBlock), 0, s, b, W, st);
+}
+void launch_ghost_z(hipStream_t s, const Box &b, FieldPtrs W, const StepState *st) {
+  const long unit = (long)b.n[0] * b.n[1] * b.m[2];
+  hipLaunchKernelGGL(k_ghost_z, dim3(grid_for(unit)), dim3(kBlock), 0, s, b, W, st);
+}
+void launch_cfl(hipStream_t s, const Box &b, FieldPtrs W, float goal, StepState *st) {
+  hipLaunchKernelGGL(k_cfl, dim3(grid_for(b.ncell)), dim3(kBlock), 0, s, b, W, goal, st);
+}
+void launch_dt(hipStream_t s, StepState *st, float goal) {
+  hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, s, st, goal);
+}
+void launch_commit(hipStream_t s, StepState *st) {
+  hipLaunchKernelGGL(k_commit, dim3(1), dim3(1), 0, s, st);
+}
+void launch_copy_state(hipStream_t s, const Box &b, FieldPtrs src, FieldPtrs dst,
+                       const StepState *st) {
+  hipLaunchKernelGGL(k_copy_state, dim3(grid_for(b.ncell)), dim3(kBlock), 0, s, b, src, dst,
+                     st);
+}
+void launch_hlld_batch(hipStream_t s, long n, float gamma, bool strict, FluxIn in,
+                       FluxOut out) {
+  if (strict)
+    launch_hlld_batch_strict(s, n, gamma, in, out);
+  else
+    hipLaunchKernelGGL(k_hlld_batch, dim3(grid_for(n)), dim3(kBlock), 0, s, n, gamma, in, out);
+}
+void launch_diag_prepare(hipStream_t s, StepState *diag, const StepState *st, float h) {
+  hipLaunchKernelGGL(k_diag_prepare, dim3(1), dim3(1), 0, s, diag, st, h);
+}
+
+}  // namespace nmhd

@@ -1,0 +1,189 @@
+"""Grid configurations and initial conditions for the benchmark workloads.
+
+A configuration is a plain dict describing one box the way the reference's
+compile-time constants do (library/simulation_config_default.h:25-122,
+application/kepler_disk_demo/simulation_config.hrb:15-74), with every
+derived float computed in FP32 exactly as those constant expressions are:
+
+  kExtent  = kGlobalExtent / Real(kGlobalSize) * Real(kSize)   (:79-81)
+  kD       = kExtent / kSize                                    (:101-103)
+  kLocalExtent = kGlobalExtent / Real(kGlobalSize) * Real(kLocalSize) (:87-89)
+
+Array layout: x fastest, shape (nz, ny, nx) per field, 8 fields in the order
+rho, vx, vy, vz, p, Bx, By, Bz; face B_d[i] on the lower d-face of cell i.
+
+Shearing box orientation follows the reference (application/
+kepler_disk_demo): the radial axis is Y (ghost layers along Y), azimuth is X,
+vertical is Z.
+
+The counter-based RNG is splitmix64 of (seed ^ (idx*8 + component)), top 24
+bits -> [0, 1); identical in every language that needs it.
+"""
+import math
+
+import numpy as np
+
+F32 = np.float32
+NFIELD = 8
+FIELDS = ("rho", "vx", "vy", "vz", "p", "bx", "by", "bz")
+
+
+def make_config(global_size, margin=(0, 0, 0), global_extent=(1.0, 1.0, 1.0),
+                gamma=5.0 / 3.0, cfl=0.4, source=0, omega=1.0, qshear=1.5,
+                shear_y=0, recon=0, integrator=0):
+    gs = [int(v) for v in global_size]
+    m = [int(v) for v in margin]
+    if m[0] != 0:
+        raise ValueError("the reference never exchanges X ghosts; margin[0] must be 0")
+    n = [gs[i] + 2 * m[i] for i in range(3)]
+    ext, d, lext = [], [], []
+    for i in range(3):
+        cell = F32(F32(global_extent[i]) / F32(gs[i]))
+        e = F32(cell * F32(n[i]))
+        ext.append(float(e))
+        d.append(float(F32(e / F32(n[i]))))
+        lext.append(float(F32(cell * F32(gs[i]))))
+    return {
+        "global_size": gs, "m": m, "n": n,
+        "global_extent": [float(F32(v)) for v in global_extent],
+        "ext": ext, "d": d, "local_ext": lext,
+        "gamma": float(F32(gamma)), "cfl": float(F32(cfl)),
+        "source": int(source), "omega": float(F32(omega)),
+        "qshear": float(F32(qshear)), "shear_y": int(shear_y),
+        "recon": int(recon), "integrator": int(integrator),
+    }
+
+
+def ncell(cfg):
+    n = cfg["n"]
+    return n[0] * n[1] * n[2]
+
+
+def interior_cells(cfg):
+    g = cfg["global_size"]
+    return g[0] * g[1] * g[2]
+
+
+def empty_state(cfg):
+    n = cfg["n"]
+    return np.zeros((NFIELD, n[2], n[1], n[0]), dtype=F32)
+
+
+# ---------------------------------------------------------------- RNG
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(x):
+    """splitmix64 finaliser of a uint64 array (wrapping arithmetic)."""
+    x = np.asarray(x, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = x + np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed, idx, component):
+    """u = (splitmix64(seed ^ (idx*8 + component)) >> 40) * 2^-24."""
+    idx = np.asarray(idx, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        key = np.uint64(seed) ^ (idx * np.uint64(8) + np.uint64(component))
+    return (splitmix64(key) >> np.uint64(40)).astype(np.float64) * (2.0 ** -24)
+
+
+# ------------------------------------------------------ initial conditions
+
+def cell_centres(cfg, axis, corner=0.0):
+    """Cell-centre coordinates along an axis of the array (ghosts included),
+    corner + d*(i + 0.5), evaluated in float64."""
+    n = cfg["n"][axis]
+    d = cfg["d"][axis]
+    return corner + d * (np.arange(n) + 0.5)
+
+
+def orszag_tang(cfg, b0=None, amp_vz=0.01):
+    """3D Orszag-Tang vortex extruded in z on [0,1)^3 (periodic, no ghosts).
+
+    rho = 25/(36 pi), p = 5/(12 pi), v = (-sin 2pi y, sin 2pi x, amp_vz sin 2pi z),
+    face B from A_z = B0 (cos 4pi x/(4pi) + cos 2pi y/(2pi)) by exact
+    differences at the cell corners, so the discrete div B is zero.
+    """
+    if b0 is None:
+        b0 = 1.0 / math.sqrt(4.0 * math.pi)
+    n = cfg["n"]
+    dx, dy, dz = (float(v) for v in cfg["d"])
+    W = empty_state(cfg)
+    x = (np.arange(n[0]) + 0.5) * dx
+    y = (np.arange(n[1]) + 0.5) * dy
+    z = (np.arange(n[2]) + 0.5) * dz
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    W[0] = 25.0 / (36.0 * math.pi)
+    W[4] = 5.0 / (12.0 * math.pi)
+    W[1] = -np.sin(2 * math.pi * Y)
+    W[2] = np.sin(2 * math.pi * X)
+    W[3] = amp_vz * np.sin(2 * math.pi * Z)
+
+    def Az(xc, yc):
+        return b0 * (np.cos(4 * math.pi * xc) / (4 * math.pi) +
+                     np.cos(2 * math.pi * yc) / (2 * math.pi))
+
+    xf = np.arange(n[0]) * dx  # lower x-faces / corners
+    yf = np.arange(n[1]) * dy
+    Zf, Yf, Xf = np.meshgrid(z, yf, xf, indexing="ij")
+    # B_x on the lower x-face (x = i dx, y = (j+1/2) dy): dAz/dy
+    W[5] = (Az(Xf, Yf + dy) - Az(Xf, Yf)) / dy
+    # B_y on the lower y-face (x = (i+1/2) dx, y = j dy): -dAz/dx
+    W[6] = -(Az(Xf + dx, Yf) - Az(Xf, Yf)) / dx
+    W[7] = 0.0
+    return W
+
+
+def orszag_tang_noisy(cfg, seed=7, amp=1e-3, b0=None):
+    """Orszag-Tang with seeded relative noise of size amp on rho, v and p from
+    the counter RNG.  The noise breaks the vortex's exact mirror symmetries,
+    under which sums of upwind velocities vanish exactly and the CT upwind
+    choice becomes a tie decided by round-off."""
+    W = orszag_tang(cfg, b0=b0)
+    n = cfg["n"]
+    idx = np.arange(n[0] * n[1] * n[2], dtype=np.uint64)
+    for comp in (0, 1, 2, 3, 4):
+        u = (2.0 * uniform01(seed, idx, comp) - 1.0).reshape(n[2], n[1], n[0])
+        if comp in (1, 2, 3):
+            W[comp] = W[comp] + amp * u
+        else:
+            W[comp] = W[comp] * (1.0 + amp * u)
+    return W
+
+
+def shearing_box_mri(cfg, seed=1002, beta=400.0, amp=0.01, rho0=1.0, p0=1.0):
+    """MRI shearing box: rho, p uniform, vertical B with plasma beta,
+    v_az = -q Omega x_rad + delta, delta ~ U[-amp, amp] per component
+    from the counter RNG.  Radial = Y (with ghosts), azimuth = X, vertical Z.
+    Ghost cells hold the same formula; the ghost fill overwrites them."""
+    n = cfg["n"]
+    W = empty_state(cfg)
+    corner_y = -cfg["ext"][1] / 2.0
+    yloc = cell_centres(cfg, 1, corner_y)  # radial position of every row
+    idx = np.arange(n[0] * n[1] * n[2], dtype=np.uint64)
+    W[0] = rho0
+    W[4] = p0
+    shear = (-cfg["qshear"] * cfg["omega"] * yloc)[None, :, None]
+    for comp in range(3):
+        dv = (2.0 * uniform01(seed, idx, comp) - 1.0) * amp
+        W[1 + comp] = dv.reshape(n[2], n[1], n[0])
+    W[1] += shear
+    W[7] = math.sqrt(2.0 * p0 / beta)
+    return W
+
+
+def config_c1(n=32):
+    """Orszag-Tang, n^3 periodic unit cube, gamma 5/3, CFL 0.4."""
+    return make_config((n, n, n), cfl=0.4)
+
+
+def config_c2(n=(256, 256, 256), cfl=0.4):
+    """Shearing box n^3 with 4 radial (Y) ghost layers per side."""
+    return make_config(n, margin=(0, 4, 0), global_extent=(1.0, 1.0, 1.0),
+                       cfl=cfl, source=1, omega=1.0, qshear=1.5, shear_y=1)

@@ -1,0 +1,41 @@
+"""The C-ABI library loads on a CPU-only host and exports every entry point
+that include/nmhd.h declares (no compute calls: no GPU here)."""
+import os
+import re
+
+from nmhd_amd import capi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names |= set(re.findall(r"\b(nmhd_[a-z_0-9]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = capi.load()
+    decl = declared_symbols()
+    assert "nmhd_create" in decl and len(decl) >= 20
+    missing = [n for n in sorted(decl) if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(capi.EXPORTS) == decl
+
+
+def test_create_rejects_bad_descriptor_without_gpu():
+    """Descriptor validation happens before any device call."""
+    from nmhd_amd import problems as P
+    cfg = P.make_config((8, 8, 8))
+    d = capi.desc_from_config(cfg)
+    d.margin[0] = 2
+    import ctypes
+    h = ctypes.c_void_p()
+    rc = capi.load().nmhd_create(ctypes.byref(d), ctypes.byref(h))
+    assert rc == 1 and not h.value
+    assert b"margin[0]" in capi.load().nmhd_last_error(None)

@@ -1,0 +1,169 @@
+"""Pin the oracle's HLLD and reconstruction (CPU only).
+
+* golden vectors written by the reference's own hlld.inl
+  (tests/golden/hlld_ref.npz, see make_hlld_golden.py): bit-for-bit;
+* the live reference build (oracle/_ref) on fresh random states when present;
+* physics that fixes the flux independently of the implementation:
+  consistency F(U, U) = F(U), the supersonic limits F_L / F_R, mirror and
+  rotation symmetry;
+* minmod MUSCL reconstruction through its defining properties.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "hlld_ref.npz")
+
+
+def physical_flux(s, gamma):
+    """Ideal-MHD flux along x of (Bx, rho, vx, vy, vz, p, By, Bz) in float64."""
+    bx, r, u, v, w, p, by, bz = (np.asarray(a, np.float64) for a in s)
+    b2 = bx * bx + by * by + bz * bz
+    pt = p + 0.5 * b2
+    e = p / (gamma - 1) + 0.5 * r * (u * u + v * v + w * w) + 0.5 * b2
+    vb = u * bx + v * by + w * bz
+    return np.stack([r * u, r * u * u + pt - bx * bx, r * v * u - bx * by,
+                     r * w * u - bx * bz, (e + pt) * u - bx * vb,
+                     by * u - bx * v, bz * u - bx * w])
+
+
+def random_states(n, seed):
+    rng = np.random.default_rng(seed)
+    x = np.zeros((15, n), np.float32)
+    x[0] = rng.uniform(-2, 2, n)
+    for i, (lo, hi) in enumerate([(0.1, 10), (-3, 3), (-3, 3), (-3, 3), (0.01, 10),
+                                  (-2, 2), (-2, 2)]):
+        x[1 + 2 * i] = rng.uniform(lo, hi, n)
+        x[2 + 2 * i] = rng.uniform(lo, hi, n)
+    return x
+
+
+def left(x):
+    return x[[0, 1, 3, 5, 7, 9, 11, 13]]
+
+
+def right(x):
+    return x[[0, 2, 4, 6, 8, 10, 12, 14]]
+
+
+def test_golden_bitwise():
+    d = np.load(GOLD)
+    x = d["states"]
+    for name, gamma in (("g53", 5.0 / 3.0), ("g75", 7.0 / 5.0)):
+        got = po.hlld_batch(x, gamma=gamma)
+        exp = d["flux_" + name]
+        assert np.array_equal(got.view(np.uint32), exp.view(np.uint32)), name
+
+
+def test_live_reference_bitwise():
+    if po.ref_lib() is None:
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    x = random_states(50000, seed=7)
+    x[0, :5000] = 0.0
+    a = po.hlld_batch(x)
+    b = po.hlld_batch(x, use_ref=True)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_consistency_equal_states():
+    x = random_states(4000, seed=3)
+    for i in range(7):
+        x[2 + 2 * i] = x[1 + 2 * i]
+    got = po.hlld_batch(x).astype(np.float64)
+    exp = physical_flux(left(x), 5.0 / 3.0)
+    scale = np.abs(exp).max(axis=1, keepdims=True) + 1.0
+    assert np.all(np.abs(got - exp) <= 2e-5 * scale)
+
+
+@pytest.mark.parametrize("shift,side", [(25.0, "L"), (-25.0, "R")])
+def test_supersonic_limits(shift, side):
+    x = random_states(4000, seed=5)
+    x[3] += shift
+    x[4] += shift
+    got = po.hlld_batch(x).astype(np.float64)
+    exp = physical_flux(left(x) if side == "L" else right(x), 5.0 / 3.0)
+    scale = np.abs(exp).max(axis=1, keepdims=True) + 1.0
+    assert np.all(np.abs(got - exp) <= 2e-5 * scale)
+
+
+def test_mirror_symmetry():
+    """x -> -x: normal components flip, left and right swap; the fluxes of
+    rho, m_t, E, B_t flip sign and the normal momentum flux does not."""
+    x = random_states(4000, seed=11)
+    m = np.empty_like(x)
+    m[0] = -x[0]
+    for i in range(7):
+        lo, hi = x[1 + 2 * i], x[2 + 2 * i]
+        sign = -1.0 if i == 1 else 1.0  # vx flips
+        m[1 + 2 * i] = sign * hi
+        m[2 + 2 * i] = sign * lo
+    f = po.hlld_batch(x).astype(np.float64)
+    g = po.hlld_batch(m).astype(np.float64)
+    expect = f * np.array([-1, 1, -1, -1, -1, -1, -1])[:, None]
+    scale = np.abs(f).max(axis=1, keepdims=True) + 1.0
+    assert np.all(np.abs(g - expect) <= 1e-5 * scale)
+
+
+def test_rotation_about_normal():
+    """(y, z) -> (z, -y) rotates the transverse fluxes the same way."""
+    x = random_states(4000, seed=13)
+    r = x.copy()
+    r[5], r[6] = x[7], x[8]        # vy' = vz
+    r[7], r[8] = -x[5], -x[6]      # vz' = -vy
+    r[11], r[12] = x[13], x[14]    # By' = Bz
+    r[13], r[14] = -x[11], -x[12]  # Bz' = -By
+    f = po.hlld_batch(x).astype(np.float64)
+    g = po.hlld_batch(r).astype(np.float64)
+    exp = np.stack([f[0], f[1], f[3], -f[2], f[4], f[6], -f[5]])
+    scale = np.abs(f).max(axis=1, keepdims=True) + 1.0
+    assert np.all(np.abs(g - exp) <= 1e-5 * scale)
+
+
+def test_bx_zero_no_rotational_jump():
+    """With Bx = 0 the Alfven waves degenerate: signBx = 0 and the transverse
+    velocity flux is continuous across the contact (no NaN, finite)."""
+    x = random_states(2000, seed=17)
+    x[0] = 0.0
+    f = po.hlld_batch(x)
+    assert np.isfinite(f).all()
+
+
+# ----------------------------------------------------------- reconstruction
+
+def test_minmod_linear_profiles_are_exact():
+    rng = np.random.default_rng(1)
+    a = rng.uniform(-5, 5, 1000).astype(np.float32)
+    s = rng.uniform(-2, 2, 1000).astype(np.float32)
+    xs = [a, a + s, a + 2 * s, a + 3 * s]
+    xs = [np.asarray(v, np.float32) for v in xs]
+    lft, rgt = po.interpolate_batch(xs)
+    mid = 0.5 * (xs[1].astype(np.float64) + xs[2].astype(np.float64))
+    assert np.allclose(lft, mid, rtol=0, atol=2e-5 * (np.abs(mid) + 1))
+    assert np.allclose(rgt, mid, rtol=0, atol=2e-5 * (np.abs(mid) + 1))
+
+
+def test_minmod_bounded_and_flat_at_extrema():
+    rng = np.random.default_rng(2)
+    xs = [rng.uniform(-1, 1, 20000).astype(np.float32) for _ in range(4)]
+    lft, rgt = po.interpolate_batch(xs)
+    x0, x1, x2, x3 = xs
+    # the left state stays between x1 and its two neighbours' range
+    lo = np.minimum(np.minimum(x0, x1), x2)
+    hi = np.maximum(np.maximum(x0, x1), x2)
+    assert np.all((lft >= lo - 1e-7) & (lft <= hi + 1e-7))
+    lo = np.minimum(np.minimum(x1, x2), x3)
+    hi = np.maximum(np.maximum(x1, x2), x3)
+    assert np.all((rgt >= lo - 1e-7) & (rgt <= hi + 1e-7))
+    ext1 = (x1 - x0) * (x2 - x1) <= 0
+    ext2 = (x2 - x1) * (x3 - x2) <= 0
+    assert np.array_equal(lft[ext1], x1[ext1])  # first order at an extremum
+    assert np.array_equal(rgt[ext2], x2[ext2])
+
+
+def test_piecewise_constant():
+    xs = [np.arange(4 * k, 4 * k + 10, dtype=np.float32) for k in range(4)]
+    lft, rgt = po.interpolate_batch(xs, recon=1)
+    assert np.array_equal(lft, xs[1]) and np.array_equal(rgt, xs[2])
