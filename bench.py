@@ -30,8 +30,8 @@ HBM_PEAK = 8.0e12
 STEP_BYTES = 160.0  # algorithmic bytes per zone-update (8 fields x 5 passes x 4 B)
 # Algorithmic bytes per interior cell of one launch of each kernel.
 KERNEL_BYTES = {
-    "stage1": 64.0,      # read W^n (8), write W* (8)
-    "stage2": 96.0,      # read W* (8) and W^n (8), write W^{n+1} (8)
+    "faces_s1": 116.0,   # face fluxes: read W (8), write 21 face fluxes
+    "faces_s2": 116.0,
     "flux_x": 60.0,      # read W (8), write 7 face fluxes
     "flux_y": 60.0,
     "flux_z": 60.0,
@@ -47,10 +47,13 @@ def parse():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, nargs=3, default=(256, 256, 256))
-    ap.add_argument("--path", choices=("auto", "unfused", "fused"), default="auto")
+    ap.add_argument("--path", choices=("auto", "unfused", "tiled", "strict"), default="auto")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="also run one device copy of this many MiB (a known-size kernel "
+                         "that calibrates HBM byte counters in a profiler run)")
     return ap.parse_args()
 
 
@@ -87,10 +90,18 @@ def main():
     from nmhd_amd import capi, problems as P
 
     torch.cuda.set_device(0)
+    if a.copy_mib:
+        src = torch.empty(a.copy_mib * (1 << 18), dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        src.fill_(1.0)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     cfg = P.config_c2(tuple(a.size))
     W = P.shearing_box_mri(cfg, seed=1002)
     path = {"auto": capi.PATH_AUTO, "unfused": capi.PATH_UNFUSED,
-            "fused": capi.PATH_FUSED}[a.path]
+            "tiled": capi.PATH_TILED, "strict": capi.PATH_STRICT}[a.path]
     ctx = capi.Context(cfg, path=path)
     ctx.set_state(W)
     goal = 1e30
@@ -141,7 +152,7 @@ def main():
                                "4 radial ghost layers/side), gamma 5/3, beta 400, CFL 0.4"
                                % (tuple(cfg["global_size"]) + tuple(cfg["n"])),
                    "global_size": cfg["global_size"], "steps": a.steps,
-                   "path": ["auto", "unfused", "fused"][ctx.get_path()]},
+                   "path": ["auto", "unfused", "tiled", "strict"][ctx.get_path()]},
         "roofline": roofline,
         "step_roofline": {"achieved": step_achieved / 1e9, "peak": HBM_PEAK / 1e9,
                           "unit": "GB/s", "frac": step_achieved / HBM_PEAK,

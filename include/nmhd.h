@@ -61,9 +61,12 @@ enum nmhd_integrator {
 };
 
 enum nmhd_path {
-  NMHD_PATH_AUTO = 0,     /* fastest path valid for the descriptor        */
-  NMHD_PATH_UNFUSED = 1,  /* per-direction flux kernels + one update pass */
-  NMHD_PATH_FUSED = 2,    /* one fused stencil kernel per RK stage        */
+  NMHD_PATH_AUTO = 0,     /* fastest path valid for the descriptor (TILED)  */
+  NMHD_PATH_UNFUSED = 1,  /* per-direction flux kernels + one update pass   */
+  /* per RK stage: one z-marching LDS-tiled kernel for all face fluxes and
+   * one cell kernel for CT + update + primitive recovery (+ the CFL
+   * reduction of the new state in the last stage) */
+  NMHD_PATH_TILED = 2,
   /* verification path: the unfused kernels with the reference's rounding
    * (no FMA contraction, IEEE division and square root, the FP64-promoted
    * HLLD literals); bit-identical to the CPU restatement of the reference */

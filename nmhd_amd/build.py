@@ -15,7 +15,8 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnmhd.so")
 
-SOURCES = ["capi.hip", "kernels_common.hip", "kernels_unfused.hip", "kernels_strict.hip"]
+SOURCES = ["capi.hip", "kernels_common.hip", "kernels_unfused.hip", "kernels_strict.hip",
+           "kernels_tiled.hip"]
 # kernels_strict.hip reproduces the reference's rounding: no FMA contraction.
 FILE_FLAGS = {"kernels_strict.hip": ["-ffp-contract=off"]}
 

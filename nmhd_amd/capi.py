@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libnmhd.so")
 
 NMHD_OK = 0
 ERRORS = {1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "EIO", 5: "ESTATE"}
-PATH_AUTO, PATH_UNFUSED, PATH_FUSED, PATH_STRICT = 0, 1, 2, 3
+PATH_AUTO, PATH_UNFUSED, PATH_TILED, PATH_STRICT = 0, 1, 2, 3
 
 # Every entry point declared in include/nmhd.h.
 EXPORTS = (

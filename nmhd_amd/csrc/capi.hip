@@ -33,11 +33,14 @@ enum Slot {
   S_UPDATE2,
   S_COPY,
   S_COMMIT,
+  S_FACES1,
+  S_FACES2,
+  S_CFL_GHOSTS,
   S_NSLOT
 };
-const char *kSlotName[S_NSLOT] = {"begin_step", "ghost_y",  "ghost_z", "cfl",
-                                  "dt",         "flux_x",   "flux_y",  "flux_z",
-                                  "update_s1",  "update_s2", "copy",   "commit"};
+const char *kSlotName[S_NSLOT] = {"begin_step", "ghost_y",   "ghost_z", "cfl",      "dt",
+                                  "flux_x",     "flux_y",    "flux_z",  "update_s1", "update_s2",
+                                  "copy",       "commit",    "faces_s1", "faces_s2", "cfl_ghosts"};
 
 struct Pending {
   int slot;
@@ -60,7 +63,8 @@ struct nmhd_ctx {
   float *scratch8[8] = {};
   StepState *st = nullptr;
   StepState *diag = nullptr;
-  int path = NMHD_PATH_UNFUSED;
+  int path = NMHD_PATH_TILED;
+  bool cfl_valid = false;  // st->cfl_next_bits holds the CFL minimum of the state
   std::string err;
   bool timing = false;
   std::vector<Pending> pending;
@@ -159,7 +163,6 @@ bool desc_ok(const nmhd_desc *d, std::string &why) {
   if (d->integrator != NMHD_MIDPOINT2 && d->integrator != NMHD_EULER1)
     return why = "unknown integrator", false;
   if (d->path < NMHD_PATH_AUTO || d->path > NMHD_PATH_STRICT) return why = "unknown path", false;
-  if (d->path == NMHD_PATH_FUSED) return why = "fused path not available in this build", false;
   long n = 1;
   for (int i = 0; i < 3; ++i) n *= (long)(d->global_size[i] + 2 * d->margin[i]);
   if (n >= (1L << 31)) return why = "box exceeds 2^31 cells", false;
@@ -195,7 +198,7 @@ void derive(const nmhd_desc *d, Box &b, nmhd_geometry &g) {
   b.recon = d->recon;
 }
 
-int resolve_path(int p) { return p == NMHD_PATH_AUTO ? NMHD_PATH_UNFUSED : p; }
+int resolve_path(int p) { return p == NMHD_PATH_AUTO ? NMHD_PATH_TILED : p; }
 
 void flux_dispatch(nmhd_ctx *c, int d, FieldPtrs W, FluxOut F, const StepState *st, int hsel) {
   if (c->path == NMHD_PATH_STRICT)
@@ -217,33 +220,49 @@ void enqueue_step(nmhd_ctx *c, float goal) {
   const Box &b = c->box;
   hipStream_t s = c->stream;
   const int A = c->cur;
-  timed(c, S_BEGIN, [&] { launch_begin_step(s, c->st); });
+  const bool tiled = c->path == NMHD_PATH_TILED;
+  const bool full_cfl = !(tiled && c->cfl_valid);
+  timed(c, S_BEGIN, [&] { launch_begin_step(s, c->st, full_cfl); });
   if (b.m[1] > 0) timed(c, S_GHOST_Y, [&] { launch_ghost_y(s, b, fields(c, A), c->st); });
   if (b.m[2] > 0) timed(c, S_GHOST_Z, [&] { launch_ghost_z(s, b, fields(c, A), c->st); });
-  timed(c, S_CFL, [&] { launch_cfl(s, b, fields(c, A), goal, c->st); });
+  if (full_cfl)
+    timed(c, S_CFL, [&] { launch_cfl(s, b, fields(c, A), goal, c->st); });
+  else
+    timed(c, S_CFL_GHOSTS, [&] { launch_cfl_ghosts(s, b, fields(c, A), c->st); });
   timed(c, S_DT, [&] { launch_dt(s, c->st, goal); });
   const FluxSet F = fluxes(c);
   const int slots[3] = {S_FLUX_X, S_FLUX_Y, S_FLUX_Z};
-  auto stage = [&](int in, int base, int out, int hsel, int slot) {
+  // one RK stage: fluxes of W[in], update of the conserved base W[base] into W[out]
+  auto stage = [&](int in, int base, int out, int hsel, bool last) {
+    if (tiled) {
+      timed(c, last ? S_FACES2 : S_FACES1,
+            [&] { launch_faces(s, b, fields(c, in), F, c->st, hsel); });
+      timed(c, last ? S_UPDATE2 : S_UPDATE1, [&] {
+        launch_update_tiled(s, b, fields(c, in), F, fields(c, base), fields(c, out), c->st, hsel,
+                            false, last);
+      });
+      return;
+    }
     for (int d = 0; d < 3; ++d)
       timed(c, slots[d], [&] { flux_dispatch(c, d, fields(c, in), F.dir[d], c->st, hsel); });
-    timed(c, slot, [&] {
+    timed(c, last ? S_UPDATE2 : S_UPDATE1, [&] {
       update_dispatch(c, fields(c, in), F, fields(c, base), fields(c, out), c->st, hsel, false);
     });
   };
   int next;
   if (c->desc.integrator == NMHD_EULER1) {
     next = (A + 1) % 3;
-    stage(A, A, next, H_FULL, S_UPDATE2);
+    stage(A, A, next, H_FULL, true);
   } else {
     const int B = (A + 1) % 3;
     next = (A + 2) % 3;
-    stage(A, A, B, H_HALF, S_UPDATE1);
-    stage(B, A, next, H_FULL, S_UPDATE2);
+    stage(A, A, B, H_HALF, false);
+    stage(B, A, next, H_FULL, true);
   }
   timed(c, S_COPY, [&] { launch_copy_state(s, b, fields(c, A), fields(c, next), c->st); });
   timed(c, S_COMMIT, [&] { launch_commit(s, c->st); });
   c->cur = next;
+  c->cfl_valid = tiled;
 }
 
 int read_step_state(nmhd_ctx *c, StepState *h) {
@@ -355,6 +374,7 @@ static int copy_state(nmhd_ctx *c, const float *const src[8], float *const dst[8
     if (!in && !dst[f]) return fail(c, NMHD_EINVAL, "null field pointer");
   }
   HIPCHECK(c, hipStreamSynchronize(c->stream));
+  if (in) c->cfl_valid = false;
   for (int f = 0; f < 8; ++f) {
     if (in)
       HIPCHECK(c, hipMemcpyAsync(c->state[c->cur][f], src[f], bytes, kind, c->stream));
@@ -391,6 +411,8 @@ int nmhd_set_time(nmhd_ctx *ctx, double t, int generation) {
   h.t_next = t;
   h.generation = generation;
   h.cfl_bits = 0x7f800000u;
+  h.cfl_next_bits = 0x7f800000u;
+  ctx->cfl_valid = false;
   return write_step_state(ctx, &h);
 }
 
@@ -443,8 +465,15 @@ int nmhd_rhs(nmhd_ctx *c, float h, float *const pace[8]) {
   for (int f = 0; f < 8; ++f) out.f[f] = c->scratch8[f];
   launch_diag_prepare(c->stream, c->diag, c->st, h);
   const FluxSet F = fluxes(c);
-  for (int d = 0; d < 3; ++d) flux_dispatch(c, d, fields(c, c->cur), F.dir[d], c->diag, H_DIAG);
-  update_dispatch(c, fields(c, c->cur), F, fields(c, c->cur), out, c->diag, H_DIAG, true);
+  if (c->path == NMHD_PATH_TILED) {
+    launch_faces(c->stream, c->box, fields(c, c->cur), F, c->diag, H_DIAG);
+    launch_update_tiled(c->stream, c->box, fields(c, c->cur), F, fields(c, c->cur), out, c->diag,
+                        H_DIAG, true, false);
+  } else {
+    for (int d = 0; d < 3; ++d)
+      flux_dispatch(c, d, fields(c, c->cur), F.dir[d], c->diag, H_DIAG);
+    update_dispatch(c, fields(c, c->cur), F, fields(c, c->cur), out, c->diag, H_DIAG, true);
+  }
   for (int f = 0; f < 8; ++f)
     HIPCHECK(c, hipMemcpyAsync(pace[f], out.f[f], bytes, hipMemcpyDeviceToHost, c->stream));
   return sync_and_check(c);
@@ -567,8 +596,8 @@ int nmhd_timing_get(nmhd_ctx *c, int i, const char **name, double *total_ms, lon
 
 int nmhd_set_path(nmhd_ctx *c, int path) {
   if (!c || path < NMHD_PATH_AUTO || path > NMHD_PATH_STRICT) return NMHD_EINVAL;
-  if (path == NMHD_PATH_FUSED) return fail(c, NMHD_EINVAL, "fused path not available in this build");
   c->path = resolve_path(path);
+  c->cfl_valid = false;
   return NMHD_OK;
 }
 

@@ -14,8 +14,12 @@
 
 namespace nmhd {
 
-__global__ void k_begin_step(StepState *st) {
-  st->cfl_bits = 0x7f800000u;  // +inf
+// Start a step.  full: the CFL minimum is rebuilt over every cell by k_cfl;
+// otherwise it starts from the stage-2 epilogue's minimum of the current state
+// and k_cfl_ghosts adds the cells the ghost fill rewrote.
+__global__ void k_begin_step(StepState *st, int full) {
+  st->cfl_bits = full ? 0x7f800000u : st->cfl_next_bits;
+  st->cfl_next_bits = 0x7f800000u;  // +inf
 }
 
 // relative_displacement (INL:288-295) with D0 = 0 and
@@ -139,13 +143,72 @@ __global__ void __launch_bounds__(256) k_cfl(Box b, FieldPtrs W, float goal, Ste
   }
 }
 
+__device__ __forceinline__ float cell_cfl(const Box &b, const FieldPtrs &W, long a, int x, int y,
+                                          int z) {
+  const float bxl = W.f[5][a], bxh = W.f[5][addr(b, wrap(x + 1, b.n[0]), y, z)];
+  const float byl = W.f[6][a], byh = W.f[6][addr(b, x, wrap(y + 1, b.n[1]), z)];
+  const float bzl = W.f[7][a], bzh = W.f[7][addr(b, x, y, wrap(z + 1, b.n[2]))];
+  const float b2 = fmaxf(bxl * bxl, bxh * bxh) + fmaxf(byl * byl, byh * byh) +
+                   fmaxf(bzl * bzl, bzh * bzh);
+  const float c = sqrtf((b.gamma * W.f[4][a] + b2) / W.f[0][a]);
+  float t = 3.0e38f;
+  for (int d = 0; d < 3; ++d) t = fminf(t, b.cfl * b.d[d] / fabsf(W.f[1 + d][a] + c));
+  return t;
+}
+
+// CFL over the cells a ghost fill rewrites: the Y ghost rows, and with Z
+// ghosts the Z ghost planes plus the last interior plane (whose upper B_z
+// face is a ghost face).  slab 0: nx * 2My * nz cells, slab 1:
+// nx * ny * (2Mz + 1) cells.
+__global__ void __launch_bounds__(256) k_cfl_ghosts(Box b, FieldPtrs W, int slab, StepState *st) {
+#pragma clang fp contract(off)
+  if (st->finished) return;
+  __shared__ float red[4];
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  const int nx = b.n[0], ny = b.n[1], nz = b.n[2];
+  float t = 3.0e38f;
+  int x = 0, y = 0, z = 0;
+  bool ok = false;
+  if (slab == 0) {
+    const int M = b.m[1];
+    const long n = (long)nx * 2 * M * nz;
+    if (i < n) {
+      x = (int)(i % nx);
+      const int r = (int)((i / nx) % (2 * M));
+      z = (int)(i / nx / (2 * M));
+      y = r < M ? r : ny - 2 * M + r;
+      ok = true;
+    }
+  } else {
+    const int M = b.m[2];
+    const long n = (long)nx * ny * (2 * M + 1);
+    if (i < n) {
+      x = (int)(i % nx);
+      y = (int)((i / nx) % ny);
+      const int r = (int)(i / nx / ny);
+      z = r < M ? r : nz - 2 * M - 1 + r;
+      ok = true;
+    }
+  }
+  if (ok) t = cell_cfl(b, W, addr(b, x, y, z), x, y, z);
+  t = wave_min(t);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) red[wid] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = red[0];
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = fminf(m, red[k]);
+    atomicMin(&st->cfl_bits, __float_as_uint(m));
+  }
+}
+
 // Decide the step: dt = CFL minimum, clipped so that t + dt <= goal.
 __global__ void k_dt(StepState *st, float goal) {
   if (st->finished) {
     st->active = 0;
     return;
   }
-  float dt = __uint_as_float(st->cfl_bits);
+  float dt = fminf(goal, __uint_as_float(st->cfl_bits));
   const double next = st->t + (double)dt;
   if (next > (double)goal) {
     dt = (float)((double)goal - st->t);
@@ -205,12 +268,23 @@ __global__ void k_diag_prepare(StepState *diag, const StepState *st, float h) {
   diag->generation = st->generation;
   diag->steps_done = 0;
   diag->goal_reached = 0;
+  diag->cfl_next_bits = 0x7f800000u;
 }
 
 // ------------------------------------------------------------- launchers
 
-void launch_begin_step(hipStream_t s, StepState *st) {
-  hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, st);
+void launch_begin_step(hipStream_t s, StepState *st, bool full) {
+  hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, st, (int)full);
+}
+void launch_cfl_ghosts(hipStream_t s, const Box &b, FieldPtrs W, StepState *st) {
+  if (b.m[1] > 0) {
+    const long n = (long)b.n[0] * 2 * b.m[1] * b.n[2];
+    hipLaunchKernelGGL(k_cfl_ghosts, dim3(grid_for(n)), dim3(kBlock), 0, s, b, W, 0, st);
+  }
+  if (b.m[2] > 0) {
+    const long n = (long)b.n[0] * b.n[1] * (2 * b.m[2] + 1);
+    hipLaunchKernelGGL(k_cfl_ghosts, dim3(grid_for(n)), dim3(kBlock), 0, s, b, W, 1, st);
+  }
 }
 void launch_ghost_y(hipStream_t s, const Box &b, FieldPtrs W, const StepState *st) {
   const long unit = (long)b.n[0] * b.m[1] * b.n[2];

@@ -48,7 +48,9 @@ struct StepState {
   int generation;
   int steps_done;
   int goal_reached;
-  int pad;
+  unsigned cfl_next_bits; // CFL minimum of the next state, reduced by the
+                          // stage-2 epilogue over the cells a ghost fill
+                          // does not touch
 };
 
 // Periodic wrap valid for offsets within [-2n, 2n) (RCo::calc_addr with
@@ -66,6 +68,16 @@ __device__ __forceinline__ long addr(const Box &b, int x, int y, int z) {
 }
 
 __device__ __forceinline__ float sq(float x) { return x * x; }
+
+// Workgroups are dispatched to the 8 XCDs round-robin (block b -> XCD b % 8),
+// and each XCD has its own L2.  Map the hardware block id to a logical tile id
+// so that XCD k processes the contiguous range of tiles
+// [k*(N/8) + min(k, N%8), ...): neighbouring tiles, which re-read each
+// other's halos, then share an L2.  The map is a bijection on [0, N).
+__device__ __forceinline__ int xcd_logical_block(int b, int N) {
+  const int k = b & 7, q = N >> 3, r = N & 7;
+  return k * q + (k < r ? k : r) + (b >> 3);
+}
 
 // Division and square root used on the hot path: the hardware reciprocal and
 // square root (1 ulp) instead of the IEEE correctly rounded sequences.

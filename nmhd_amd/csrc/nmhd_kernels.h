@@ -33,7 +33,8 @@ inline unsigned grid_for(long n, int block = kBlock) {
 void host_displacement(const Box &b, double t, float *normalized, int *n, float *minor);
 
 // kernels_common.hip
-void launch_begin_step(hipStream_t s, StepState *st);
+void launch_begin_step(hipStream_t s, StepState *st, bool full);
+void launch_cfl_ghosts(hipStream_t s, const Box &b, FieldPtrs W, StepState *st);
 void launch_ghost_y(hipStream_t s, const Box &b, FieldPtrs W, const StepState *st);
 void launch_ghost_z(hipStream_t s, const Box &b, FieldPtrs W, const StepState *st);
 void launch_cfl(hipStream_t s, const Box &b, FieldPtrs W, float goal, StepState *st);
@@ -58,5 +59,13 @@ void launch_flux(hipStream_t s, int dir, const Box &b, FieldPtrs W, FluxOut F,
 void launch_update(hipStream_t s, const Box &b, FieldPtrs Win, const FluxSet &F,
                    FieldPtrs Wbase, FieldPtrs Wout, const StepState *st, int hsel,
                    bool pace_only);
+
+// kernels_tiled.hip
+int tiled_zchunk(const Box &b);
+void launch_faces(hipStream_t s, const Box &b, FieldPtrs W, const FluxSet &F, const StepState *st,
+                  int hsel);
+void launch_update_tiled(hipStream_t s, const Box &b, FieldPtrs Win, const FluxSet &F,
+                         FieldPtrs Wbase, FieldPtrs Wout, StepState *st, int hsel, bool pace_only,
+                         bool cfl);
 
 }  // namespace nmhd

@@ -150,11 +150,15 @@ def test_strict_shearing_box_20_steps_bitwise(gpu_ready):
 
 # ------------------------------------------------ fast path: tolerance
 
+FAST_PATHS = [capi.PATH_TILED, capi.PATH_UNFUSED]
+
+
+@pytest.mark.parametrize("path", FAST_PATHS)
 @pytest.mark.parametrize("name", ["otn16", "sb16"])
-def test_fast_rhs(gpu_ready, name):
-    announce("fast rhs " + name)
+def test_fast_rhs(gpu_ready, name, path):
+    announce("fast rhs %s path %d" % (name, path))
     cfg, W = case(name)
-    o, g = both(cfg, W)
+    o, g = both(cfg, W, path=path)
     a = g.rhs(0.003)
     b = o.rhs(0.003)
     err = group_rel_err(a, b, PACE_GROUPS)
@@ -165,11 +169,12 @@ def test_fast_rhs(gpu_ready, name):
     assert err.max() < 1e-4
 
 
-@pytest.mark.parametrize("name", ["otn16", "sb16"])
-def test_fast_steps(gpu_ready, name):
-    announce("fast 5 steps " + name)
+@pytest.mark.parametrize("path", FAST_PATHS)
+@pytest.mark.parametrize("name", ["otn16", "sb16", "sb24x16x8"])
+def test_fast_steps(gpu_ready, name, path):
+    announce("fast 5 steps %s path %d" % (name, path))
     cfg, W = case(name)
-    o, g = both(cfg, W)
+    o, g = both(cfg, W, path=path)
     for k in range(5):
         dto, _ = o.step()
         dtg, _ = g.step()
@@ -227,6 +232,52 @@ def test_fast_div_b_per_step(gpu_ready):
         a = g.get_state()
         bmax = np.abs(a[5:8]).max()
         assert np.abs(div_b(a, cfg["d"])).max() <= 1e-6 * bmax / cfg["d"][0], k
+
+
+def test_tiled_matches_strict_rhs_closely(gpu_ready):
+    """The tiled kernels and the strict kernels evaluate the same right-hand
+    side (different rounding only)."""
+    announce("tiled vs strict rhs")
+    cfg, W = case("sb24x16x8")
+    a = capi.Context(cfg, path=capi.PATH_TILED)
+    b = capi.Context(cfg, path=capi.PATH_STRICT)
+    a.set_state(W)
+    b.set_state(W)
+    err = group_rel_err(a.rhs(0.002), b.rhs(0.002), PACE_GROUPS)
+    assert err.max() < 1e-4
+
+
+def test_tiled_enqueued_run_equals_single_steps(gpu_ready):
+    """The tiled path reduces the CFL time of step n+1 in the last stage of
+    step n (plus a pass over the refilled ghosts); a synchronous single step
+    recomputes it over the whole box.  Both must give the same run."""
+    announce("tiled incremental CFL vs full CFL")
+    cfg, W = case("sb24x16x8")
+    a = capi.Context(cfg, path=capi.PATH_TILED)
+    b = capi.Context(cfg, path=capi.PATH_TILED)
+    a.set_state(W)
+    b.set_state(W)
+    k, _ = a.run(1e30, 6)
+    for _ in range(6):
+        b.step()
+    assert k == 6
+    assert a.get_time() == b.get_time()
+    assert bitwise_equal(a.get_state(), b.get_state())
+
+
+@pytest.mark.parametrize("shape", [(16, 16, 16), (40, 24, 12), (8, 8, 1), (70, 6, 5)])
+def test_tiled_odd_shapes(gpu_ready, shape):
+    """Tiles that straddle the box edge, thin boxes and a 2D box."""
+    announce("tiled odd shape %s" % (shape,))
+    cfg = P.make_config(shape, cfl=0.4)
+    W = P.orszag_tang_noisy(cfg)
+    o, g = both(cfg, W, path=capi.PATH_TILED)
+    for k in range(3):
+        o.step()
+        g.step()
+    a, b = g.get_state(), o.get_state()
+    assert group_frac_within(a, b, 3e-5) >= 0.999
+    assert group_rel_err(a, b).max() < 1e-3
 
 
 # ------------------------------------------------------------ stepping
