@@ -33,6 +33,25 @@ def _run(cmd):
     subprocess.check_call(cmd)
 
 
+def build_cpp():
+    """The C++ facade's programs: the example driver and the C++ tests."""
+    inc = ["-I", os.path.join(ROOT, "include")]
+    link = ["-L", HERE, "-lnmhd", "-L/opt/rocm/lib", "-Wl,-rpath-link,/opt/rocm/lib"]
+    bindir = os.path.join(HERE, "bin")
+    tbin = os.path.join(ROOT, "tests", "cpp", "bin")
+    os.makedirs(bindir, exist_ok=True)
+    os.makedirs(tbin, exist_ok=True)
+    flags = ["g++", "-std=c++14", "-O2", "-Wall"]
+    _run(flags + inc + [os.path.join(ROOT, "tests", "cpp", "crystal_test.cpp"),
+                        "-o", os.path.join(tbin, "crystal_test")])
+    _run(flags + inc + [os.path.join(ROOT, "tests", "cpp", "facade_test.cpp"),
+                        "-o", os.path.join(tbin, "facade_test")] + link +
+         ["-Wl,-rpath,$ORIGIN/../../../nmhd_amd"])
+    _run(flags + inc + [os.path.join(HERE, "apps", "kepler_disk.cpp"),
+                        "-o", os.path.join(bindir, "kepler_disk")] + link +
+         ["-Wl,-rpath,$ORIGIN/.."])
+
+
 def build(jobs=8, extra=()):
     os.makedirs(OBJ, exist_ok=True)
     objs = []
@@ -45,6 +64,7 @@ def build(jobs=8, extra=()):
     with ThreadPoolExecutor(max_workers=jobs) as ex:
         list(ex.map(_run, cmds))
     _run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", *objs, "-o", LIB])
+    build_cpp()
     return LIB
 
 

@@ -1,0 +1,56 @@
+"""The C++ facade (include/nmhd/): addressing laws on the CPU, and the
+MHDSolver drop-in (proceed / save_state / load_state / read / write) on the
+GPU through libnmhd.so.  The programs are built by __graft_entry__.build()."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "tests", "cpp", "bin")
+OUT = os.path.join(ROOT, "gpurun_out")
+
+
+def run(exe, *args, timeout=120):
+    path = os.path.join(BIN, exe) if os.sep not in exe else exe
+    if not os.path.exists(path):
+        pytest.fail("%s not built (run __graft_entry__.build())" % path)
+    return subprocess.run([path, *args], capture_output=True, text=True, timeout=timeout)
+
+
+def test_crystal_addressing_laws():
+    r = run("crystal_test")
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_facade_header_compiles_standalone(tmp_path):
+    src = tmp_path / "inc.cpp"
+    src.write_text('#include "nmhd/mhd_solver.h"\nint main() { return 0; }\n')
+    r = subprocess.run(["g++", "-std=c++14", "-fsyntax-only", "-I",
+                        os.path.join(ROOT, "include"), str(src)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_facade_restart_and_io(gpu_ready, tmp_path):
+    print("\n[case] C++ facade restart / io", flush=True)
+    r = run("facade_test", str(tmp_path))
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+
+
+@pytest.mark.gpu
+def test_kepler_disk_app_continue(gpu_ready, tmp_path):
+    """The shearing-box driver: a run stopped by its step limit saves the
+    'c' checkpoint and a --continue run resumes from it."""
+    print("\n[case] kepler_disk app", flush=True)
+    app = os.path.join(ROOT, "nmhd_amd", "bin", "kepler_disk")
+    r1 = run(app, "--new", "-o", str(tmp_path), "-n", "16", "--steps", "3")
+    assert r1.returncode == 0, r1.stdout + r1.stderr
+    assert "ToBeContinued" in r1.stdout and "generation = 3" in r1.stdout
+    r2 = run(app, "--continue", "-o", str(tmp_path), "-n", "16", "--steps", "2")
+    assert r2.returncode == 0, r2.stdout + r2.stderr
+    assert "generation = 5" in r2.stdout
+    assert os.path.getsize(os.path.join(str(tmp_path), "c", "0000.bin")) == 12 + 8 * 4 * 16 * 24 * 16

@@ -1,0 +1,314 @@
+"""Pin the oracle's full step (CPU only) by what the mathematics fixes:
+conservation, constrained transport, invariance of uniform states, second-
+order convergence to an exact nonlinear solution, mirror symmetry, the
+closed forms of the CFL time and the shearing displacement, the shear remap
+as cyclic shifts, and the equivalence of ghost-margin copies with plain
+periodic wrap."""
+import math
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from mhd_checks import bitwise_equal, div_b, group_rel_err, totals
+from nmhd_amd import problems as P
+
+
+# ------------------------------------------------------------- helpers
+
+def cp_alfven(n, axis, amp=0.1, rho=1.0, p=0.1):
+    """Circularly polarised Alfven wave travelling along `axis` at
+    v_A = B_par / sqrt(rho) = 1: an exact nonlinear solution of ideal MHD.
+    Box: n cells along the axis, one cell across."""
+    size = [1, 1, 1]
+    size[axis] = n
+    cfg = P.make_config(size, cfl=0.4)
+    W = P.empty_state(cfg)
+    s = (np.arange(n) + 0.5) / n
+    ph = 2 * math.pi * s
+    t1, t2 = (axis + 1) % 3, (axis + 2) % 3
+    shape = [1, 1, 1]
+    shape[2 - axis] = n  # arrays are (z, y, x)
+    b1 = (amp * np.sin(ph)).reshape(shape)
+    b2 = (amp * np.cos(ph)).reshape(shape)
+    W[0] = rho
+    W[4] = p
+    W[5 + axis] = 1.0
+    W[5 + t1] = b1
+    W[5 + t2] = b2
+    W[1 + t1] = -b1 / math.sqrt(rho)
+    W[1 + t2] = -b2 / math.sqrt(rho)
+    return cfg, W
+
+
+def mirror_x(W):
+    """x -> -x: reverse cells along x, negate v_x and B_x, and move face B_x
+    from lower face i to lower face n-i (the mirror of the upper face)."""
+    M = W[:, :, :, ::-1].copy()
+    M[1] = -M[1]
+    bx = W[5]
+    M[5] = -np.roll(bx[:, :, ::-1], 1, axis=2)
+    return M
+
+
+def sb_config(nx=16, ny=16, nz=8, shear=1):
+    return P.make_config((nx, ny, nz), margin=(0, 4, 0), cfl=0.4, source=shear,
+                         shear_y=shear)
+
+
+# ------------------------------------------------------- conservation
+
+def test_conservation_and_div_b_periodic():
+    cfg = P.config_c1(16)
+    W = P.orszag_tang_noisy(cfg)
+    o = po.Oracle(cfg, W)
+    t0 = totals(W, cfg["gamma"])
+    o.run(1e30, 10)
+    S = o.get_state()
+    t1 = totals(S, cfg["gamma"])
+    scale = np.array([t0[0], t0[0], t0[0], t0[0], t0[4], 1, 1, 1]) * 1.0
+    assert np.all(np.abs(t1 - t0) <= 2e-6 * np.maximum(np.abs(scale), 1e-3))
+    bmax = np.abs(S[5:8]).max()
+    assert np.abs(div_b(S, cfg["d"])).max() <= 1e-6 * bmax / cfg["d"][0]
+
+
+def test_uniform_state_is_steady():
+    cfg = P.make_config((8, 8, 8), cfl=0.4)
+    W = P.empty_state(cfg)
+    for i, v in enumerate([1.3, 0.2, -0.3, 0.1, 0.7, 0.4, -0.5, 0.25]):
+        W[i] = v
+    o = po.Oracle(cfg, W)
+    pace = o.rhs(0.0)
+    assert np.all(pace == 0.0)  # identical fluxes on every face cancel exactly
+    o.run(1e30, 3)
+    err = group_rel_err(o.get_state(), W)
+    assert err.max() < 2e-6  # only the conserved<->primitive round trip rounds
+
+
+def test_cfl_closed_form_v_plus_c():
+    """dt = C dx / |v_d + sqrt((gamma p + b2)/rho)| with b2 the face maxima:
+    |v + c|, not |v| + c (mhd_solver.inl:715-720)."""
+    cfg = P.make_config((4, 4, 4), cfl=0.4)
+    W = P.empty_state(cfg)
+    rho, p, bz = 2.0, 1.0, 0.5
+    c = math.sqrt((cfg["gamma"] * p + bz * bz) / rho)
+    W[0], W[4], W[7] = rho, p, bz
+    W[1] = W[2] = W[3] = -0.5 * c  # |v + c| = c/2 on every axis (|v| + c = 1.5 c)
+    o = po.Oracle(cfg, W)
+    dt = o.cfl_dt(1e30)
+    assert dt == pytest.approx(0.4 * 0.25 / (0.5 * c), rel=1e-6)
+    W[1] = W[2] = W[3] = 0.5 * c  # |v + c| = 1.5 c
+    o.set_state(W)
+    assert o.cfl_dt(1e30) == pytest.approx(0.4 * 0.25 / (1.5 * c), rel=1e-6)
+    assert o.cfl_dt(1e-3) == np.float32(1e-3)  # never above the goal time
+
+
+# ------------------------------------------------------- convergence
+
+def line_box(n, axis):
+    size = [1, 1, 1]
+    size[axis] = n
+    cfg = P.make_config(size, cfl=0.4)
+    shape = [1, 1, 1]
+    shape[2 - axis] = n  # arrays are (z, y, x)
+    s = np.sin(2 * math.pi * (np.arange(n) + 0.5) / n).reshape(shape)
+    return cfg, s
+
+
+def coarsen(a, axis):
+    ax = 2 - axis
+    sl0 = [slice(None)] * 3
+    sl1 = [slice(None)] * 3
+    sl0[ax] = slice(0, None, 2)
+    sl1[ax] = slice(1, None, 2)
+    return 0.5 * (a[tuple(sl0)] + a[tuple(sl1)])
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_entropy_wave_converges_second_order(axis):
+    """Density wave advected at v = 1 for one period (exact solution: the
+    initial state): minmod MUSCL + HLLD + midpoint RK2 converge at ~1.7."""
+    errs = []
+    for n in (32, 64, 128):
+        cfg, s = line_box(n, axis)
+        W = P.empty_state(cfg)
+        W[0] = 1 + 1e-2 * s
+        W[1 + axis] = 1.0
+        W[4] = 1.0
+        o = po.Oracle(cfg, W)
+        assert o.run(1.0, 100000)[1]
+        errs.append(np.abs(o.get_state()[0] - W[0]).mean())
+    orders = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert min(orders) > 1.5, (errs, orders)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_sound_wave_self_convergence(axis):
+    """Finite-amplitude sound wave (steepening, so compared N against 2N)."""
+    sol = {}
+    for n in (32, 64, 128, 256):
+        cfg, s = line_box(n, axis)
+        W = P.empty_state(cfg)
+        g = cfg["gamma"]
+        W[0] = 1 + 1e-2 * s
+        W[1 + axis] = math.sqrt(g) * 1e-2 * s
+        W[4] = 1 + g * 1e-2 * s
+        o = po.Oracle(cfg, W)
+        o.run(0.5, 100000)
+        sol[n] = o.get_state()[0].astype(np.float64)
+    errs = [np.abs(sol[n] - coarsen(sol[2 * n], axis)).mean() for n in (32, 64, 128)]
+    orders = [math.log2(errs[i] / errs[i + 1]) for i in range(2)]
+    assert min(orders) > 1.5, (errs, orders)
+
+
+@pytest.mark.parametrize("axis", [0, 1, 2])
+def test_alfven_wave_self_convergence(axis):
+    """Circularly polarised Alfven wave (beta 2): the solution converges, at
+    about first order, because the reference's CT takes the EMF of the upwind
+    face outright (mhd_solver.inl:829-835) for half of every face update."""
+    sol = {}
+    for n in (32, 64, 128, 256):
+        cfg, W = cp_alfven(n, axis, amp=0.1, p=1.0)
+        o = po.Oracle(cfg, W)
+        o.run(0.5, 100000)
+        sol[n] = o.get_state()[5 + (axis + 1) % 3].astype(np.float64)
+    errs = [np.abs(sol[n] - coarsen(sol[2 * n], axis)).mean() for n in (32, 64, 128)]
+    assert errs[0] > errs[1] > errs[2], errs
+    assert math.log2(errs[1] / errs[2]) > 0.5, errs
+
+
+def test_whole_step_mirror_symmetry():
+    cfg = P.config_c1(16)
+    W = P.orszag_tang_noisy(cfg, seed=3)
+    a = po.Oracle(cfg, W)
+    b = po.Oracle(cfg, mirror_x(W))
+    # the CFL speed |v_x + c| is itself not mirror symmetric (the reference's
+    # quirk), so both runs take the same goal-clipped dt
+    assert a.cfl_dt(1e30) != b.cfl_dt(1e30)
+    da, _ = a.step(goal=0.005)
+    db, _ = b.step(goal=0.005)
+    assert da == db == np.float32(0.005)
+    err = group_rel_err(mirror_x(b.get_state()), a.get_state())
+    assert err.max() < 1e-5
+
+
+def test_euler_and_midpoint_differ_at_second_order():
+    """One step of euler1 and midpoint2 differ by dt * (P(U*) - P(U)) = O(dt^2):
+    halving dt divides the difference by ~4."""
+    cfg = P.config_c1(16)
+    W = P.orszag_tang(cfg)
+    cfg_e = dict(cfg, integrator=1)
+    diffs = []
+    for dt in (2e-3, 1e-3, 5e-4):
+        a = po.Oracle(cfg, W)
+        b = po.Oracle(cfg_e, W)
+        a.step(goal=dt)
+        b.step(goal=dt)  # both clipped to dt
+        diffs.append(np.abs(a.get_state() - b.get_state()).astype(np.float64).sum())
+    r1, r2 = diffs[0] / diffs[1], diffs[1] / diffs[2]
+    assert 3.5 < r1 < 4.5 and 3.5 < r2 < 4.5, (diffs, r1, r2)
+
+
+# ------------------------------------------------ shearing displacement
+
+def test_relative_displacement_closed_forms():
+    cfg = sb_config(256, 256, 8)
+    assert po.relative_displacement(cfg, 0.0) == (0.0, 0, 0.0)
+    # D = -1.5 t folded into [0, 1): t = 1/3 -> D = 0.5 -> 128 cells exactly
+    nd, n, minor = po.relative_displacement(cfg, 1.0 / 3.0)
+    assert n == 127 or n == 128
+    assert nd == pytest.approx(0.5, abs=1e-6)
+    assert n + minor == pytest.approx(128.0, abs=1e-4)
+    # t = 0.1 -> D = 0.85 -> 217.6 cells
+    nd, n, minor = po.relative_displacement(cfg, 0.1)
+    assert n == 217 and minor == pytest.approx(0.6, abs=1e-4)
+
+
+def unique_state(cfg):
+    n = cfg["n"]
+    W = P.empty_state(cfg)
+    ids = np.arange(n[0] * n[1] * n[2], dtype=np.float64).reshape(n[2], n[1], n[0])
+    for f in range(8):
+        W[f] = (f * 1e5 + ids) / 64.0  # exact in float32
+    return W
+
+
+@pytest.mark.parametrize("t", [0.0, 0.1, 0.37])
+def test_shear_fill_is_a_cyclic_shift(t):
+    """Lower ghost rows are the top interior rows shifted by +n along X, upper
+    ghost rows the bottom interior rows shifted by -n-1, with v_X -/+ relvel;
+    the outer By face of the first upper ghost row is kept."""
+    cfg = sb_config(16, 16, 4)
+    W = unique_state(cfg)
+    o = po.Oracle(cfg, W)
+    o.set_time(t)
+    o.fill_ghosts()
+    S = o.get_state()
+    _, n, _ = po.relative_displacement(cfg, t)
+    M, ny = 4, cfg["n"][1]
+    relvel = -cfg["qshear"] * cfg["omega"] * cfg["local_ext"][1]
+    for f in range(8):
+        lower = np.roll(W[f][:, ny - 2 * M:ny - M, :], -n, axis=2)
+        upper = np.roll(W[f][:, M:2 * M, :], n + 1, axis=2)
+        if f == 1:
+            lower = lower - relvel
+            upper = upper + relvel
+        tol = 4 * np.spacing(np.abs(W[f]).max().astype(np.float32))
+        assert np.allclose(S[f][:, :M, :], lower, rtol=0, atol=tol), f
+        if f == 6:  # By: the outer face (first upper ghost row) is kept
+            assert np.array_equal(S[f][:, ny - M, :], W[f][:, ny - M, :])
+            assert np.allclose(S[f][:, ny - M + 1:, :], upper[:, 1:, :], rtol=0, atol=tol)
+        else:
+            assert np.allclose(S[f][:, ny - M:, :], upper, rtol=0, atol=tol), f
+        # interior untouched
+        assert np.array_equal(S[f][:, M:ny - M, :], W[f][:, M:ny - M, :])
+
+
+def test_shear_fill_keeps_azimuthally_uniform_fields():
+    cfg = sb_config(16, 16, 4)
+    W = P.shearing_box_mri(cfg, amp=0.0)  # exact shear flow, uniform along X
+    o = po.Oracle(cfg, W)
+    o.set_time(0.23)
+    o.fill_ghosts()
+    S = o.get_state()
+    assert np.allclose(S, W, rtol=0, atol=1e-6)  # ghost v_X continues the shear profile
+
+
+# --------------------------------------------- margins vs periodic wrap
+
+def pad_periodic(W, axis_arr, m):
+    """Add m ghost layers on both sides of array axis `axis_arr` (0 = z,
+    1 = y) as periodic copies of the interior."""
+    n = W.shape[1 + axis_arr]
+    idx = np.arange(-m, n + m) % n
+    return np.take(W, idx, axis=1 + axis_arr)
+
+
+@pytest.mark.parametrize("axis", ["y", "z"])
+def test_margin_copies_equal_periodic_wrap(axis):
+    """Periodic ghost copies (Y with the By outer-face rule, Z plain) give the
+    same interior as wrapping the interior itself, bit for bit."""
+    base = P.config_c1(16)
+    W = P.orszag_tang_noisy(base, seed=11)
+    if axis == "y":
+        cfg = P.make_config((16, 16, 16), margin=(0, 4, 0), cfl=0.4)
+        Wm = pad_periodic(W, 1, 4)
+        sl = (slice(None), slice(None), slice(4, 20), slice(None))
+    else:
+        cfg = P.make_config((16, 16, 16), margin=(0, 0, 4), cfl=0.4)
+        Wm = pad_periodic(W, 0, 4)
+        sl = (slice(None), slice(4, 20), slice(None), slice(None))
+    a = po.Oracle(base, W)
+    b = po.Oracle(cfg, Wm)
+    for _ in range(3):
+        da, _ = a.step()
+        db, _ = b.step()
+        assert da == db
+    assert bitwise_equal(b.get_state()[sl], a.get_state())
+
+
+def test_goal_clipping_ends_on_goal():
+    cfg = P.config_c1(8)
+    o = po.Oracle(cfg, P.orszag_tang(cfg))
+    k, reached = o.run(0.05, 1000)
+    assert reached and abs(o.time - 0.05) < 1e-7 and o.generation == k
