@@ -38,11 +38,11 @@ def counters(db):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("trace")
-    ap.add_argument("--pmc", nargs="*", default=[])
+    ap.add_argument("--pmc", nargs="+", action="append", default=[])
     a = ap.parse_args()
     times = kernel_times(a.trace)
     pmc = {}
-    for db in a.pmc:
+    for db in [d for group in a.pmc for d in group]:
         for k, d in counters(db).items():
             pmc.setdefault(k, {}).update(d)
     names = sorted(times, key=lambda k: -times[k][0] * times[k][1])
