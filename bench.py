@@ -15,7 +15,19 @@ The JSON line also carries
                   against the 8 TB/s HBM peak;
   step_roofline -- the whole step: 160 algorithmic bytes per zone-update;
   cpu_baseline -- the oracle (CPU restatement of the reference step) on the
-                  GPU box's host cores, a 2-step sample of the same box.
+                  GPU box's host cores, a 2-step sample of the same box;
+  batched      -- 64 periodic boxes of uneven sizes (edges from
+                  {16,24,32,48,64,96}, seed 1004; Orszag-Tang and blast
+                  states) advanced 100 steps with per-box dt by the batched
+                  path (persistent tile launches, HIP-graph replay, autotuned
+                  tiles), against the same boxes advanced one after another
+                  by single-box contexts;
+  adjoint      -- the hand-written adjoint on the 64^3 Orszag-Tang box with
+                  a smooth density perturbation (seed 1005): 64 steps with dt
+                  recorded from the forward pass, loss = magnetic energy,
+                  checkpoints every 8 steps; backward/forward time ratio and
+                  central finite differences (eps 1e-3) along 32 smooth
+                  random directions (seed 1006).
 """
 import argparse
 import json
@@ -51,6 +63,13 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--profile-steps", type=int, default=10)
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batch-boxes", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=100)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-size", type=int, default=64)
+    ap.add_argument("--adjoint-steps", type=int, default=64)
+    ap.add_argument("--fd-directions", type=int, default=32)
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="also run one device copy of this many MiB (a known-size kernel "
                          "that calibrates HBM byte counters in a profiler run)")
@@ -76,6 +95,132 @@ def cpu_baseline(cfg, W, steps):
         "sample": "%d steps of the same %dx%dx%d shearing box on the oracle "
                   "(C restatement of the reference step, OpenMP, %d threads)"
                   % ((steps,) + tuple(cfg["global_size"]) + (threads,)),
+    }
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_shearing_box_256.json")
+
+
+def measured_traffic(kernel, size, path):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (None when the summary
+    does not cover this size, path and kernel)."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if list(t.get("global_size", [])) != list(size) or t.get("path") != path:
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k["bytes_per_launch"]
+
+
+def batched_leg(a, torch):
+    from nmhd_amd import capi, problems as P
+    boxes = P.batch_boxes(seed=1004, nbox=a.batch_boxes)
+    cells = sum(P.interior_cells(c) for c, _ in boxes)
+    B = capi.Batch([c for c, _ in boxes])
+    for i, (_, W) in enumerate(boxes):
+        B.set_state(i, W)
+    t0 = time.perf_counter()
+    timed_shapes = B.autotune()
+    tune_s = time.perf_counter() - t0
+    B.run(1e30, a.warmup, use_graph=True)  # the full-CFL first step, graph capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    kmin, _ = B.run(1e30, a.batch_steps, use_graph=True)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    assert kmin == a.batch_steps, kmin
+    batched = cells * a.batch_steps / el
+    launch = B.get_launch()
+    B.close()
+    ctxs = []
+    for cfg, W in boxes:
+        g = capi.Context(cfg, path=capi.PATH_TILED)
+        g.set_state(W)
+        g.run(1e30, a.warmup)
+        ctxs.append(g)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for g in ctxs:
+        k, _ = g.run(1e30, a.batch_steps)
+        assert k == a.batch_steps
+    torch.cuda.synchronize()
+    el2 = time.perf_counter() - t0
+    for g in ctxs:
+        g.close()
+    plain = cells * a.batch_steps / el2
+    return {
+        "value": batched, "unit": "zone-updates/s", "boxes": len(boxes), "cells": cells,
+        "distinct_shapes": len({tuple(c["global_size"]) for c, _ in boxes}),
+        "steps": a.batch_steps, "ms_per_step": el / a.batch_steps * 1e3,
+        "plain_sequential": plain, "plain_ms_per_step": el2 / a.batch_steps * 1e3,
+        "vs_plain_sequential": batched / plain,
+        "launch": {"faces_zchunk": launch[0], "update_zchunk": launch[1],
+                   "faces_wg_per_cu": launch[2], "update_wg_per_cu": launch[3]},
+        "autotune_s": tune_s, "autotune_timed_shapes": timed_shapes,
+        "workload": "64 periodic boxes, edges i.i.d. from {16,24,32,48,64,96} (seed 1004), "
+                    "Orszag-Tang (B0 ~ U[0.5,1.5]/sqrt(4 pi)) and blast states, per-box dt, "
+                    "HIP-graph replay; baseline: the same boxes stepped one after another by "
+                    "single-box contexts (tiled path)",
+    }
+
+
+def adjoint_leg(a, torch):
+    import numpy as np
+    from nmhd_amd import adjoint as A, problems as P
+    cfg = P.config_c1(a.adjoint_size)
+    W = P.adjoint_case(cfg, seed=1005)
+    S = A.Stepper(cfg)
+    Wc = torch.tensor(W, dtype=torch.float32, device="cuda")
+    with torch.no_grad():
+        dts = A.record_dts(Wc, S, a.adjoint_steps)
+    # warm-up of both passes
+    W0 = Wc.clone().requires_grad_(True)
+    WN, _ = A.rollout(W0, S, dts=dts, checkpoint="uniform", every=8)
+    (0.5 * (WN[5:8] ** 2).sum()).backward()
+    torch.cuda.synchronize()
+    W0 = Wc.clone().requires_grad_(True)
+    t0 = time.perf_counter()
+    WN, _ = A.rollout(W0, S, dts=dts, checkpoint="uniform", every=8)
+    loss = 0.5 * (WN[5:8].double() ** 2).sum()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    grad = W0.grad.double()
+
+    def run_loss(Wi):
+        X = Wi
+        for dt in dts:
+            X = S.forward(X, dt)
+        return float(0.5 * (X[5:8].double() ** 2).sum())
+
+    eps = 1e-3
+    rel = []
+    for i in range(a.fd_directions):
+        D = P.smooth_direction(cfg, 1006, i)
+        D[0] *= W[0].mean()
+        Dt = torch.tensor(D, dtype=torch.float64, device="cuda")
+        fd = (run_loss((Wc + eps * Dt).float().contiguous()) -
+              run_loss((Wc - eps * Dt).float().contiguous())) / (2 * eps)
+        ad = float((grad * Dt).sum())
+        rel.append(abs(fd - ad) / max(abs(ad), 1e-300))
+    zones = P.interior_cells(cfg) * a.adjoint_steps
+    return {
+        "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3,
+        "backward_over_forward": (t2 - t1) / (t1 - t0),
+        "forward_zone_updates_per_s": zones / (t1 - t0),
+        "adjoint_zone_updates_per_s": zones / (t2 - t1),
+        "fd_directions": a.fd_directions, "fd_eps": eps,
+        "fd_rel_err_max": float(np.max(rel)), "fd_rel_err_median": float(np.median(rel)),
+        "workload": "%d^3 Orszag-Tang + 8 Fourier density modes (|k|<=4, 1e-2, seed 1005), "
+                    "%d steps, dt recorded from the forward pass, loss = 1/2 sum B^2, "
+                    "checkpoints every 8 steps (backward includes recomputation); FD along "
+                    "smooth directions in rho, v (seed 1006)" % (a.adjoint_size, a.adjoint_steps),
     }
 
 
@@ -130,8 +275,10 @@ def main():
     dom = max(kern, key=lambda k: kern[k][0])
     mean_s = kern[dom][0] / kern[dom][1] / 1e3
     achieved = KERNEL_BYTES[dom] * zones / mean_s
+    path_name = ["auto", "unfused", "tiled", "strict"][ctx.get_path()]
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
-                "unit": "GB/s", "frac": achieved / HBM_PEAK, "traffic": None,
+                "unit": "GB/s", "frac": achieved / HBM_PEAK,
+                "traffic": measured_traffic(dom, cfg["global_size"], path_name),
                 "kernel": dom, "mean_launch_ms": mean_s * 1e3,
                 "algorithmic_bytes_per_launch": KERNEL_BYTES[dom] * zones}
     step_achieved = value * STEP_BYTES
@@ -152,7 +299,7 @@ def main():
                                "4 radial ghost layers/side), gamma 5/3, beta 400, CFL 0.4"
                                % (tuple(cfg["global_size"]) + tuple(cfg["n"])),
                    "global_size": cfg["global_size"], "steps": a.steps,
-                   "path": ["auto", "unfused", "tiled", "strict"][ctx.get_path()]},
+                   "path": path_name},
         "roofline": roofline,
         "step_roofline": {"achieved": step_achieved / 1e9, "peak": HBM_PEAK / 1e9,
                           "unit": "GB/s", "frac": step_achieved / HBM_PEAK,
@@ -160,6 +307,8 @@ def main():
         "kernel_ms_per_step": {k: v[0] / a.profile_steps for k, v in tim.items() if v[1]},
     }
     ctx.close()
+    out["batched"] = None if a.no_batched else batched_leg(a, torch)
+    out["adjoint"] = None if a.no_adjoint else adjoint_leg(a, torch)
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(cfg, W, a.cpu_steps)
     else:

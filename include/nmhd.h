@@ -188,6 +188,103 @@ int nmhd_timing_get(nmhd_ctx *ctx, int i, const char **name, double *total_ms,
 int nmhd_set_path(nmhd_ctx *ctx, int path);
 int nmhd_get_path(const nmhd_ctx *ctx, int *path);
 
+/* ------------------------------------------------------------------------
+ * Batches: many independent periodic boxes of uneven sizes advanced
+ * together.  No reference counterpart: the reference runs one
+ * compile-time-sized box per process (mhd_solver.h:514-640), so its
+ * equivalent is one MHDSolver::proceed (INL:1086-1204) per box, run one
+ * after another.  Every box keeps its own time, CFL dt and goal clipping;
+ * its results are bit-identical to the same box advanced alone by an
+ * nmhd_ctx on NMHD_PATH_TILED.  Each RK stage is two persistent launches
+ * over a work table of tiles from all boxes; runs replay the step as a HIP
+ * graph.  Boxes must be periodic: margin {0,0,0}, source NONE, shear_y 0,
+ * and all boxes share one integrator.
+ * ---------------------------------------------------------------------- */
+typedef struct nmhd_batch nmhd_batch;
+
+/* Tile shapes (TX x TY columns marched along z) of the batched kernels. */
+enum nmhd_tile {
+  NMHD_TILE_64x4 = 0,
+  NMHD_TILE_32x8 = 1,
+  NMHD_TILE_16x16 = 2
+};
+
+int nmhd_batch_create(const nmhd_desc *descs, int nbox, nmhd_batch **out);
+int nmhd_batch_destroy(nmhd_batch *batch);
+/* Message of the last failure on batch (or of the last failed create when
+ * batch is NULL). */
+const char *nmhd_batch_last_error(const nmhd_batch *batch);
+int nmhd_batch_get_geometry(const nmhd_batch *batch, int box, nmhd_geometry *geo);
+
+/* State and time of one box, as nmhd_set_state / nmhd_set_time. */
+int nmhd_batch_set_state(nmhd_batch *batch, int box, const float *const fields[8]);
+int nmhd_batch_get_state(nmhd_batch *batch, int box, float *const fields[8]);
+int nmhd_batch_set_time(nmhd_batch *batch, int box, double t, int generation);
+int nmhd_batch_get_time(nmhd_batch *batch, int box, double *t, int *generation);
+/* Steps taken by a box in the last nmhd_batch_run, whether it reached the
+ * goal, and its last dt. */
+int nmhd_batch_get_status(nmhd_batch *batch, int box, int *steps_taken, int *goal_reached,
+                          float *last_dt);
+
+/* Tiling: tile shape (enum nmhd_tile) of box's face and update kernels
+ * (box = -1: every box); z chunk (planes per work item) and workgroups per
+ * CU of the two persistent launches.  Results do not depend on the tiling. */
+int nmhd_batch_set_tiles(nmhd_batch *batch, int box, int faces_tile, int update_tile);
+int nmhd_batch_set_launch(nmhd_batch *batch, int faces_zchunk, int update_zchunk,
+                          int faces_wg_per_cu, int update_wg_per_cu);
+int nmhd_batch_get_launch(const nmhd_batch *batch, int *faces_zchunk, int *update_zchunk,
+                          int *faces_wg_per_cu, int *update_wg_per_cu);
+/* Autotuner: per distinct box shape, time every tile shape of both kernels
+ * (one workgroup walking that shape's tiles, i.e. the workgroup-time the
+ * shape costs in a persistent launch) and keep the fastest; then time the
+ * z chunk and workgroups per CU of the whole batch.  Results are cached in
+ * the text file cache_path (NULL: $NMHD_TUNE_CACHE, else
+ * $HOME/.cache/nmhd_amd/batch_tuning.txt), keyed by the GPU architecture
+ * and the box shape (and, for the launch parameters, the batch's shapes);
+ * cached entries are reused without timing.  *timed_shapes (may be NULL)
+ * returns how many shapes were timed.  Overwrites the batch's scratch
+ * buffers, not the boxes' states. */
+int nmhd_batch_autotune(nmhd_batch *batch, const char *cache_path, int *timed_shapes);
+
+/* MHDSolver::proceed for every box: at most nsteps steps toward goal_time
+ * each (a box that reaches goal_time stops there).  use_graph != 0 replays
+ * the step as a HIP graph (the first step of a run after set_state or
+ * set_time is always enqueued directly, with a full CFL pass).
+ * Synchronises; *min_steps = fewest steps taken by any box, *all_reached =
+ * every box reached goal_time (either may be NULL). */
+int nmhd_batch_run(nmhd_batch *batch, float goal_time, int nsteps, int use_graph,
+                   int *min_steps, int *all_reached);
+
+/* ------------------------------------------------------------------------
+ * Adjoint: the step S(W; dt) of a periodic box with dt frozen (no CFL), and
+ * its hand-written reverse lam_in = (dS/dW)^T lam_out.  No reference
+ * counterpart (the reference has no derivatives); S is one iteration of
+ * proceed()'s loop (INL:1138-1199) for margin {0,0,0}, source NONE.  State
+ * and adjoint arrays are caller-owned DEVICE arrays (8 fields, ncell floats
+ * each, same layout as nmhd_set_state); work is enqueued on the stream set
+ * by nmhd_adj_set_stream (default: the object's own) and is asynchronous
+ * unless stated.  The reverse of the minmod limiter, of the upwind CT choice
+ * and of the HLLD fan follow the branches the forward takes at w_in.
+ * ---------------------------------------------------------------------- */
+typedef struct nmhd_adj nmhd_adj;
+
+int nmhd_adj_create(const nmhd_desc *desc, nmhd_adj **out);
+int nmhd_adj_destroy(nmhd_adj *adj);
+const char *nmhd_adj_last_error(const nmhd_adj *adj);
+int nmhd_adj_get_geometry(const nmhd_adj *adj, nmhd_geometry *geo);
+/* stream: a hipStream_t (e.g. torch.cuda.current_stream().cuda_stream), or
+ * NULL for the object's own stream. */
+int nmhd_adj_set_stream(nmhd_adj *adj, void *stream);
+int nmhd_adj_synchronize(nmhd_adj *adj);
+/* The CFL dt of a device state (as nmhd_cfl_dt).  Synchronises. */
+int nmhd_adj_cfl_dt(nmhd_adj *adj, const float *const w[8], float goal_time, float *dt);
+/* w_out = S(w_in; dt) with the production stage kernels. */
+int nmhd_adj_forward(nmhd_adj *adj, const float *const w_in[8], float dt,
+                     float *const w_out[8]);
+/* lam_in = (dS/dW at w_in)^T lam_out; recomputes the stage states from w_in. */
+int nmhd_adj_reverse(nmhd_adj *adj, const float *const w_in[8], float dt,
+                     const float *const lam_out[8], float *const lam_in[8]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,259 @@
+"""Differentiable time stepping: the hand-written adjoint (nmhd_adj_* in
+include/nmhd.h) behind torch.autograd.
+
+A thin PyTorch-ROCm binding: torch owns the device arrays and the stream;
+the library's kernels run on torch's current stream through the C ABI, so
+no copies or extra synchronisation are needed.
+
+    stepper = Stepper(cfg)                      # periodic box, source none
+    W0 = torch.tensor(state, device="cuda", requires_grad=True)
+    WN, dts = rollout(W0, stepper, nsteps=64, checkpoint="uniform", every=8)
+    loss = 0.5 * (WN[5:8] ** 2).sum()           # magnetic energy
+    loss.backward()                             # W0.grad via the adjoint
+
+dt is frozen: the forward pass records each step's CFL dt (or takes the
+given ones), and the backward pass reuses them.  Checkpointing keeps a
+subset of the states of the forward pass and recomputes the others in the
+backward pass:
+  "all"      every state (memory N, no recomputation);
+  "uniform"  every `every`-th state (default round(sqrt(N))): memory
+             N/every + every, one recomputed forward pass;
+  "revolve"  binomial checkpointing with `snapshots` slots (default
+             ceil(log2 N)): the split points minimise the number of
+             recomputed steps for that memory (the offline optimum that
+             Griewank's revolve schedule attains), found by dynamic
+             programming over (steps, slots).
+"""
+import ctypes
+import functools
+import math
+
+import torch
+
+from . import capi
+
+_FPA = ctypes.c_void_p * 8
+
+
+def _sig(L):
+    if getattr(L, "_adj_sig", False):
+        return L
+    V = ctypes.c_void_p
+    FP = ctypes.POINTER(ctypes.c_float)
+    sig = {
+        "nmhd_adj_create": [ctypes.POINTER(capi.NmhdDesc), ctypes.POINTER(V)],
+        "nmhd_adj_destroy": [V],
+        "nmhd_adj_get_geometry": [V, ctypes.POINTER(capi.NmhdGeometry)],
+        "nmhd_adj_set_stream": [V, V],
+        "nmhd_adj_synchronize": [V],
+        "nmhd_adj_cfl_dt": [V, _FPA, ctypes.c_float, FP],
+        "nmhd_adj_forward": [V, _FPA, ctypes.c_float, _FPA],
+        "nmhd_adj_reverse": [V, _FPA, ctypes.c_float, _FPA, _FPA],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = ctypes.c_int
+    L.nmhd_adj_last_error.argtypes = [V]
+    L.nmhd_adj_last_error.restype = ctypes.c_char_p
+    L._adj_sig = True
+    return L
+
+
+def _lib():
+    return _sig(capi.load())
+
+
+def _ptrs(W):
+    if not (W.is_cuda and W.dtype == torch.float32 and W.is_contiguous()):
+        raise ValueError("state arrays must be contiguous float32 CUDA tensors")
+    return _FPA(*[W[i].data_ptr() for i in range(8)])
+
+
+class Stepper:
+    """S(W; dt) of one periodic box and its reverse, on torch's stream."""
+
+    def __init__(self, cfg):
+        L = _lib()
+        self.cfg = cfg
+        self._desc = capi.desc_from_config(cfg)
+        h = ctypes.c_void_p()
+        self._check(L.nmhd_adj_create(ctypes.byref(self._desc), ctypes.byref(h)), None)
+        self.h = h
+        g = capi.NmhdGeometry()
+        self._check(L.nmhd_adj_get_geometry(self.h, ctypes.byref(g)), self.h)
+        self.shape = (8, g.size[2], g.size[1], g.size[0])
+
+    @staticmethod
+    def _check(rc, h):
+        if rc != 0:
+            msg = _lib().nmhd_adj_last_error(h)
+            raise capi.NmhdError("nmhd adjoint error %s: %s"
+                                 % (capi.ERRORS.get(rc, rc), msg.decode() if msg else ""))
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib().nmhd_adj_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _on_torch_stream(self):
+        s = torch.cuda.current_stream().cuda_stream
+        self._check(_lib().nmhd_adj_set_stream(self.h, ctypes.c_void_p(s)), self.h)
+
+    def _shape_ok(self, W):
+        if tuple(W.shape) != self.shape:
+            raise ValueError("state shape %s != %s" % (tuple(W.shape), self.shape))
+
+    def cfl_dt(self, W, goal=1e30):
+        self._shape_ok(W)
+        self._on_torch_stream()
+        dt = ctypes.c_float()
+        self._check(_lib().nmhd_adj_cfl_dt(self.h, _ptrs(W), goal, ctypes.byref(dt)), self.h)
+        return dt.value
+
+    def forward(self, W, dt):
+        self._shape_ok(W)
+        self._on_torch_stream()
+        out = torch.empty_like(W)
+        self._check(_lib().nmhd_adj_forward(self.h, _ptrs(W), float(dt), _ptrs(out)), self.h)
+        return out
+
+    def reverse(self, W, dt, lam_out):
+        self._shape_ok(W)
+        self._shape_ok(lam_out)
+        self._on_torch_stream()
+        lam_out = lam_out.contiguous()
+        lam_in = torch.empty_like(W)
+        self._check(_lib().nmhd_adj_reverse(self.h, _ptrs(W), float(dt), _ptrs(lam_out),
+                                            _ptrs(lam_in)), self.h)
+        return lam_in
+
+
+# ------------------------------------------------------------ checkpointing
+
+@functools.lru_cache(maxsize=None)
+def revolve_cost(steps, slots):
+    """Fewest forward steps needed to reverse `steps` steps when the first
+    state is held and `slots` more states may be stored at once."""
+    if steps <= 1:
+        return 0
+    if slots == 0:
+        return steps * (steps - 1) // 2
+    return min(m + revolve_cost(steps - m, slots - 1) + revolve_cost(m, slots)
+               for m in range(1, steps))
+
+
+@functools.lru_cache(maxsize=None)
+def revolve_split(steps, slots):
+    best, arg = None, 1
+    for m in range(1, steps):
+        c = m + revolve_cost(steps - m, slots - 1) + revolve_cost(m, slots)
+        if best is None or c < best:
+            best, arg = c, m
+    return arg
+
+
+def reverse_segment(stepper, W_a, a, b, dts, lam, slots, counter=None):
+    """Reverse steps a..b-1 given the state W_a = W^a; returns dL/dW^a."""
+    fwd = stepper.forward
+    if b - a == 1:
+        return stepper.reverse(W_a, dts[a], lam)
+    if slots == 0:
+        for n in range(b - 1, a - 1, -1):
+            W = W_a
+            for j in range(a, n):
+                W = fwd(W, dts[j])
+                if counter is not None:
+                    counter[0] += 1
+            lam = stepper.reverse(W, dts[n], lam)
+        return lam
+    m = revolve_split(b - a, slots)
+    W = W_a
+    for j in range(a, a + m):
+        W = fwd(W, dts[j])
+        if counter is not None:
+            counter[0] += 1
+    lam = reverse_segment(stepper, W, a + m, b, dts, lam, slots - 1, counter)
+    del W
+    return reverse_segment(stepper, W_a, a, a + m, dts, lam, slots, counter)
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, W0, stepper, dts, checkpoint, every, snapshots):
+        n = len(dts)
+        keep = {0: W0.detach()}
+        W = W0.detach()
+        for k in range(n):
+            W = stepper.forward(W, dts[k])
+            if checkpoint == "all" or (checkpoint == "uniform" and (k + 1) % every == 0
+                                       and k + 1 < n):
+                keep[k + 1] = W
+        ctx.stepper, ctx.dts, ctx.keep = stepper, list(dts), keep
+        ctx.checkpoint, ctx.every, ctx.snapshots = checkpoint, every, snapshots
+        return W
+
+    @staticmethod
+    def backward(ctx, gN):
+        stepper, dts, keep = ctx.stepper, ctx.dts, ctx.keep
+        n = len(dts)
+        lam = gN.contiguous()
+        if ctx.checkpoint == "revolve":
+            lam = reverse_segment(stepper, keep[0], 0, n, dts, lam, ctx.snapshots)
+        else:
+            starts = sorted(keep)
+            for i in range(len(starts) - 1, -1, -1):
+                a = starts[i]
+                b = starts[i + 1] if i + 1 < len(starts) else n
+                states = [keep[a]]
+                for j in range(a, b - 1):
+                    states.append(stepper.forward(states[-1], dts[j]))
+                for j in range(b - 1, a - 1, -1):
+                    lam = stepper.reverse(states[j - a], dts[j], lam)
+                del states
+        ctx.keep = None
+        return lam, None, None, None, None, None
+
+
+def record_dts(W0, stepper, nsteps, goal=1e30):
+    """The CFL dt of every step of the forward trajectory from W0."""
+    dts = []
+    W = W0.detach()
+    t = 0.0
+    for _ in range(nsteps):
+        dt = stepper.cfl_dt(W, goal)
+        if t + dt > goal:
+            dt = float(goal - t)
+        dts.append(dt)
+        W = stepper.forward(W, dt)
+        t += dt
+    return dts
+
+
+def rollout(W0, stepper, nsteps=None, dts=None, checkpoint="uniform", every=None,
+            snapshots=None):
+    """W_N = S(...S(W0; dt_0)...; dt_{N-1}), differentiable w.r.t. W0.
+
+    dts: the step sizes (frozen); when None they are recorded from the CFL
+    condition along the trajectory (an extra forward pass without grad).
+    Returns (W_N, dts)."""
+    if dts is None:
+        if nsteps is None:
+            raise ValueError("give nsteps or dts")
+        with torch.no_grad():
+            dts = record_dts(W0, stepper, nsteps)
+    n = len(dts)
+    if checkpoint not in ("all", "uniform", "revolve"):
+        raise ValueError("checkpoint must be 'all', 'uniform' or 'revolve'")
+    if every is None:
+        every = max(1, int(round(math.sqrt(n))))
+    if snapshots is None:
+        snapshots = max(1, int(math.ceil(math.log2(max(n, 2)))))
+    return _Rollout.apply(W0.contiguous(), stepper, tuple(dts), checkpoint, every,
+                          snapshots), list(dts)

@@ -15,8 +15,9 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libnmhd.so")
 
-SOURCES = ["capi.hip", "kernels_common.hip", "kernels_unfused.hip", "kernels_strict.hip",
-           "kernels_tiled.hip"]
+SOURCES = ["capi.hip", "capi_batch.hip", "capi_adjoint.hip", "kernels_common.hip",
+           "kernels_unfused.hip", "kernels_strict.hip", "kernels_tiled.hip", "kernels_batch.hip",
+           "kernels_adjoint.hip"]
 # kernels_strict.hip reproduces the reference's rounding: no FMA contraction.
 FILE_FLAGS = {"kernels_strict.hip": ["-ffp-contract=off"]}
 
@@ -50,6 +51,10 @@ def build_cpp():
     _run(flags + inc + [os.path.join(HERE, "apps", "kepler_disk.cpp"),
                         "-o", os.path.join(bindir, "kepler_disk")] + link +
          ["-Wl,-rpath,$ORIGIN/.."])
+    # host (double) build of the HLLD reverse sweep for the CPU derivative tests
+    _run(["hipcc", "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", "-fPIC", "-shared",
+          "-I", CSRC, os.path.join(ROOT, "tests", "cpp", "adj_hlld_host.cpp"),
+          "-o", os.path.join(tbin, "libadj_hlld_host.so")])
 
 
 def build(jobs=8, extra=()):

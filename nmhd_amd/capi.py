@@ -24,7 +24,16 @@ EXPORTS = (
     "nmhd_step", "nmhd_run", "nmhd_reset_run", "nmhd_enqueue_steps",
     "nmhd_synchronize", "nmhd_hlld_batch", "nmhd_timing_enable",
     "nmhd_timing_reset", "nmhd_timing_get", "nmhd_set_path", "nmhd_get_path",
+    "nmhd_batch_create", "nmhd_batch_destroy", "nmhd_batch_last_error",
+    "nmhd_batch_get_geometry", "nmhd_batch_set_state", "nmhd_batch_get_state",
+    "nmhd_batch_set_time", "nmhd_batch_get_time", "nmhd_batch_get_status",
+    "nmhd_batch_set_tiles", "nmhd_batch_set_launch", "nmhd_batch_get_launch",
+    "nmhd_batch_autotune", "nmhd_batch_run",
+    "nmhd_adj_create", "nmhd_adj_destroy", "nmhd_adj_last_error", "nmhd_adj_get_geometry",
+    "nmhd_adj_set_stream", "nmhd_adj_synchronize", "nmhd_adj_cfl_dt", "nmhd_adj_forward",
+    "nmhd_adj_reverse",
 )
+TILE_64x4, TILE_32x8, TILE_16x16 = 0, 1, 2
 
 
 class NmhdDesc(ctypes.Structure):
@@ -99,6 +108,19 @@ def load():
                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)],
         "nmhd_set_path": [_VP, ctypes.c_int],
         "nmhd_get_path": [_VP, _IP],
+        "nmhd_batch_create": [ctypes.POINTER(NmhdDesc), ctypes.c_int, ctypes.POINTER(_VP)],
+        "nmhd_batch_destroy": [_VP],
+        "nmhd_batch_get_geometry": [_VP, ctypes.c_int, ctypes.POINTER(NmhdGeometry)],
+        "nmhd_batch_set_state": [_VP, ctypes.c_int, _FPP],
+        "nmhd_batch_get_state": [_VP, ctypes.c_int, _FPP],
+        "nmhd_batch_set_time": [_VP, ctypes.c_int, ctypes.c_double, ctypes.c_int],
+        "nmhd_batch_get_time": [_VP, ctypes.c_int, ctypes.POINTER(ctypes.c_double), _IP],
+        "nmhd_batch_get_status": [_VP, ctypes.c_int, _IP, _IP, _FP],
+        "nmhd_batch_set_tiles": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "nmhd_batch_set_launch": [_VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+        "nmhd_batch_get_launch": [_VP, _IP, _IP, _IP, _IP],
+        "nmhd_batch_autotune": [_VP, ctypes.c_char_p, _IP],
+        "nmhd_batch_run": [_VP, ctypes.c_float, ctypes.c_int, ctypes.c_int, _IP, _IP],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -106,6 +128,8 @@ def load():
         fn.restype = ctypes.c_int
     L.nmhd_last_error.argtypes = [_VP]
     L.nmhd_last_error.restype = ctypes.c_char_p
+    L.nmhd_batch_last_error.argtypes = [_VP]
+    L.nmhd_batch_last_error.restype = ctypes.c_char_p
     _lib = L
     return L
 
@@ -118,9 +142,10 @@ def _ptrs(arrays):
     return arr
 
 
-def _check(rc, handle=None):
+def _check(rc, handle=None, batch=False):
     if rc != NMHD_OK:
-        msg = load().nmhd_last_error(handle)
+        L = load()
+        msg = (L.nmhd_batch_last_error if batch else L.nmhd_last_error)(handle)
         raise NmhdError("nmhd error %s: %s" % (ERRORS.get(rc, rc),
                                                msg.decode() if msg else ""))
 
@@ -285,3 +310,97 @@ def hlld_batch(inputs, gamma=5.0 / 3.0, strict=False):
     _check(load().nmhd_hlld_batch(n, gamma, PATH_STRICT if strict else PATH_AUTO,
                                   _ptrs(ins), _ptrs(outs)))
     return np.stack(outs)
+
+
+class Batch:
+    """Many independent periodic boxes advanced together (nmhd_batch_*):
+    per-box time and dt, persistent tile launches, HIP-graph replay."""
+
+    def __init__(self, cfgs):
+        L = load()
+        self.cfgs = list(cfgs)
+        descs = (NmhdDesc * len(self.cfgs))()
+        for i, c in enumerate(self.cfgs):
+            descs[i] = desc_from_config(c)
+        h = _VP()
+        _check(L.nmhd_batch_create(descs, len(self.cfgs), ctypes.byref(h)), None, batch=True)
+        self.h = h
+        self.nbox = len(self.cfgs)
+        self.shapes, self.ncells = [], []
+        for i in range(self.nbox):
+            g = NmhdGeometry()
+            self._c(L.nmhd_batch_get_geometry(self.h, i, ctypes.byref(g)))
+            self.shapes.append((g.size[2], g.size[1], g.size[0]))
+            self.ncells.append(g.ncell)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().nmhd_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc):
+        _check(rc, self.h, batch=True)
+
+    def set_state(self, box, W):
+        arrs = [np.ascontiguousarray(W[i], dtype=np.float32).reshape(-1) for i in range(8)]
+        for a in arrs:
+            if a.size != self.ncells[box]:
+                raise ValueError("field size %d != box size %d" % (a.size, self.ncells[box]))
+        self._c(load().nmhd_batch_set_state(self.h, box, _ptrs(arrs)))
+
+    def get_state(self, box):
+        out = np.zeros((8,) + self.shapes[box], np.float32)
+        self._c(load().nmhd_batch_get_state(self.h, box,
+                                            _ptrs([out[i].reshape(-1) for i in range(8)])))
+        return out
+
+    def set_time(self, box, t, generation=0):
+        self._c(load().nmhd_batch_set_time(self.h, box, t, generation))
+
+    def get_time(self, box):
+        t = ctypes.c_double()
+        g = ctypes.c_int()
+        self._c(load().nmhd_batch_get_time(self.h, box, ctypes.byref(t), ctypes.byref(g)))
+        return t.value, g.value
+
+    def status(self, box):
+        """(steps taken in the last run, goal reached, last dt)"""
+        k = ctypes.c_int()
+        r = ctypes.c_int()
+        dt = ctypes.c_float()
+        self._c(load().nmhd_batch_get_status(self.h, box, ctypes.byref(k), ctypes.byref(r),
+                                             ctypes.byref(dt)))
+        return k.value, bool(r.value), dt.value
+
+    def set_tiles(self, faces_tile, update_tile, box=-1):
+        self._c(load().nmhd_batch_set_tiles(self.h, box, faces_tile, update_tile))
+
+    def set_launch(self, faces_zchunk, update_zchunk, faces_wg_per_cu, update_wg_per_cu):
+        self._c(load().nmhd_batch_set_launch(self.h, faces_zchunk, update_zchunk,
+                                             faces_wg_per_cu, update_wg_per_cu))
+
+    def get_launch(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        self._c(load().nmhd_batch_get_launch(self.h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def autotune(self, cache_path=None):
+        """Returns the number of box shapes that had to be timed."""
+        n = ctypes.c_int()
+        p = cache_path.encode() if cache_path else None
+        self._c(load().nmhd_batch_autotune(self.h, p, ctypes.byref(n)))
+        return n.value
+
+    def run(self, goal, nsteps, use_graph=True):
+        """(fewest steps taken by any box, every box reached goal)"""
+        k = ctypes.c_int()
+        r = ctypes.c_int()
+        self._c(load().nmhd_batch_run(self.h, goal, nsteps, int(use_graph), ctypes.byref(k),
+                                      ctypes.byref(r)))
+        return k.value, bool(r.value)

@@ -14,9 +14,70 @@
 #include <vector>
 
 #include "../../include/nmhd.h"
-#include "nmhd_kernels.h"
+#include "capi_internal.h"
 
 using namespace nmhd;
+
+namespace nmhd {
+
+bool desc_valid(const nmhd_desc *d, std::string &why) {
+  for (int i = 0; i < 3; ++i) {
+    if (d->global_size[i] <= 0) return why = "global_size must be positive", false;
+    if (d->margin[i] < 0) return why = "margin must be >= 0", false;
+    if (!(d->global_extent[i] > 0.0f)) return why = "global_extent must be positive", false;
+  }
+  if (d->margin[0] != 0) return why = "margin[0] must be 0 (X ghosts are never exchanged)", false;
+  if (d->margin[1] > 0 && d->global_size[1] < d->margin[1])
+    return why = "Y interior must hold at least margin[1] rows", false;
+  if (d->margin[2] > 0 && d->global_size[2] < d->margin[2])
+    return why = "Z interior must hold at least margin[2] planes", false;
+  if (!(d->gamma > 1.0f)) return why = "gamma must be > 1", false;
+  if (!(d->cfl_factor > 0.0f)) return why = "cfl_factor must be positive", false;
+  if (d->source != NMHD_SOURCE_NONE && d->source != NMHD_SOURCE_SHEARING_BOX)
+    return why = "unknown source", false;
+  if (d->shear_y && d->margin[1] <= 0)
+    return why = "shearing-periodic Y fill needs margin[1] > 0", false;
+  if (d->recon != NMHD_RECON_MINBEE2 && d->recon != NMHD_RECON_PIECEWISECONST1)
+    return why = "unknown recon", false;
+  if (d->integrator != NMHD_MIDPOINT2 && d->integrator != NMHD_EULER1)
+    return why = "unknown integrator", false;
+  if (d->path < NMHD_PATH_AUTO || d->path > NMHD_PATH_STRICT) return why = "unknown path", false;
+  long n = 1;
+  for (int i = 0; i < 3; ++i) n *= (long)(d->global_size[i] + 2 * d->margin[i]);
+  if (n >= (1L << 31)) return why = "box exceeds 2^31 cells", false;
+  return true;
+}
+
+// Derived constants, in FP32 exactly as the reference's constant
+// expressions (simulation_config_default.h:79-103).
+void derive_box(const nmhd_desc *d, Box &b, nmhd_geometry &g) {
+  std::memset(&b, 0, sizeof(b));
+  std::memset(&g, 0, sizeof(g));
+  for (int i = 0; i < 3; ++i) {
+    const int n = d->global_size[i] + 2 * d->margin[i];
+    const float cell = d->global_extent[i] / (float)d->global_size[i];
+    const float ext = cell * (float)n;
+    b.n[i] = g.size[i] = n;
+    b.m[i] = d->margin[i];
+    b.ext[i] = g.extent[i] = ext;
+    b.d[i] = g.d[i] = ext / (float)n;
+    b.inv_d[i] = (float)(1.0 / (double)b.d[i]);
+    b.local_ext[i] = g.local_extent[i] = cell * (float)d->global_size[i];
+  }
+  b.ncell = (long)b.n[0] * b.n[1] * b.n[2];
+  g.ncell = (size_t)b.ncell;
+  b.gamma = d->gamma;
+  b.gm1 = d->gamma - 1.0f;
+  b.inv_gm1 = (float)(1.0 / (double)b.gm1);
+  b.cfl = d->cfl_factor;
+  b.source = d->source;
+  b.omega = d->omega;
+  b.qshear = d->qshear;
+  b.shear_y = d->shear_y;
+  b.recon = d->recon;
+}
+
+}  // namespace nmhd
 
 namespace {
 
@@ -141,63 +202,6 @@ void collect_timing(nmhd_ctx *c) {
   c->pending.clear();
 }
 
-bool desc_ok(const nmhd_desc *d, std::string &why) {
-  for (int i = 0; i < 3; ++i) {
-    if (d->global_size[i] <= 0) return why = "global_size must be positive", false;
-    if (d->margin[i] < 0) return why = "margin must be >= 0", false;
-    if (!(d->global_extent[i] > 0.0f)) return why = "global_extent must be positive", false;
-  }
-  if (d->margin[0] != 0) return why = "margin[0] must be 0 (X ghosts are never exchanged)", false;
-  if (d->margin[1] > 0 && d->global_size[1] < d->margin[1])
-    return why = "Y interior must hold at least margin[1] rows", false;
-  if (d->margin[2] > 0 && d->global_size[2] < d->margin[2])
-    return why = "Z interior must hold at least margin[2] planes", false;
-  if (!(d->gamma > 1.0f)) return why = "gamma must be > 1", false;
-  if (!(d->cfl_factor > 0.0f)) return why = "cfl_factor must be positive", false;
-  if (d->source != NMHD_SOURCE_NONE && d->source != NMHD_SOURCE_SHEARING_BOX)
-    return why = "unknown source", false;
-  if (d->shear_y && d->margin[1] <= 0)
-    return why = "shearing-periodic Y fill needs margin[1] > 0", false;
-  if (d->recon != NMHD_RECON_MINBEE2 && d->recon != NMHD_RECON_PIECEWISECONST1)
-    return why = "unknown recon", false;
-  if (d->integrator != NMHD_MIDPOINT2 && d->integrator != NMHD_EULER1)
-    return why = "unknown integrator", false;
-  if (d->path < NMHD_PATH_AUTO || d->path > NMHD_PATH_STRICT) return why = "unknown path", false;
-  long n = 1;
-  for (int i = 0; i < 3; ++i) n *= (long)(d->global_size[i] + 2 * d->margin[i]);
-  if (n >= (1L << 31)) return why = "box exceeds 2^31 cells", false;
-  return true;
-}
-
-// Derived constants, in FP32 exactly as the reference's constant
-// expressions (simulation_config_default.h:79-103).
-void derive(const nmhd_desc *d, Box &b, nmhd_geometry &g) {
-  std::memset(&b, 0, sizeof(b));
-  std::memset(&g, 0, sizeof(g));
-  for (int i = 0; i < 3; ++i) {
-    const int n = d->global_size[i] + 2 * d->margin[i];
-    const float cell = d->global_extent[i] / (float)d->global_size[i];
-    const float ext = cell * (float)n;
-    b.n[i] = g.size[i] = n;
-    b.m[i] = d->margin[i];
-    b.ext[i] = g.extent[i] = ext;
-    b.d[i] = g.d[i] = ext / (float)n;
-    b.inv_d[i] = (float)(1.0 / (double)b.d[i]);
-    b.local_ext[i] = g.local_extent[i] = cell * (float)d->global_size[i];
-  }
-  b.ncell = (long)b.n[0] * b.n[1] * b.n[2];
-  g.ncell = (size_t)b.ncell;
-  b.gamma = d->gamma;
-  b.gm1 = d->gamma - 1.0f;
-  b.inv_gm1 = (float)(1.0 / (double)b.gm1);
-  b.cfl = d->cfl_factor;
-  b.source = d->source;
-  b.omega = d->omega;
-  b.qshear = d->qshear;
-  b.shear_y = d->shear_y;
-  b.recon = d->recon;
-}
-
 int resolve_path(int p) { return p == NMHD_PATH_AUTO ? NMHD_PATH_TILED : p; }
 
 void flux_dispatch(nmhd_ctx *c, int d, FieldPtrs W, FluxOut F, const StepState *st, int hsel) {
@@ -314,10 +318,10 @@ int nmhd_create(const nmhd_desc *desc, nmhd_ctx **out) {
   if (!desc || !out) return fail(nullptr, NMHD_EINVAL, "null argument");
   *out = nullptr;
   std::string why;
-  if (!desc_ok(desc, why)) return fail(nullptr, NMHD_EINVAL, why);
+  if (!desc_valid(desc, why)) return fail(nullptr, NMHD_EINVAL, why);
   nmhd_ctx *c = new nmhd_ctx();
   c->desc = *desc;
-  derive(desc, c->box, c->geo);
+  derive_box(desc, c->box, c->geo);
   c->path = resolve_path(desc->path);
   auto bail = [&](int code, const std::string &msg) {
     free_all(c);

@@ -1,0 +1,258 @@
+// capi_adjoint.hip -- the nmhd_adj_* entry points of include/nmhd.h: the
+// forward step with a frozen dt and its hand-written reverse, on caller-owned
+// device state arrays (e.g. torch tensors), launched on a caller-chosen
+// stream.  The reverse step recomputes the stage states it needs (W*, W^{n+1})
+// from W^n with the forward kernels, then runs the stage reverses
+// (kernels_adjoint.hip) from the last stage to the first.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+
+#include "capi_internal.h"
+
+using namespace nmhd;
+
+namespace {
+std::string g_adj_create_error;
+constexpr int kWork = 5 + 3 + 3 + 3 + 3 + 14;  // gU, gB, lE, lbb, abn, face
+}  // namespace
+
+struct nmhd_adj {
+  nmhd_desc desc;
+  Box box;
+  nmhd_geometry geo;
+  hipStream_t own = nullptr;
+  hipStream_t stream = nullptr;
+  float *wstar[8] = {};   // recomputed stage-1 state
+  float *wnext[8] = {};   // recomputed stage-2 state
+  float *lstar[8] = {};   // dL/dW*
+  float *flux[21] = {};   // face fluxes of the forward stages
+  float *work[kWork] = {};
+  float *scratch_out[8] = {};
+  StepState *st = nullptr;
+  std::string err;
+};
+
+namespace {
+
+int afail(nmhd_adj *a, int code, const std::string &msg) {
+  if (a)
+    a->err = msg;
+  else
+    g_adj_create_error = msg;
+  return code;
+}
+
+#define AHIP(a, call)                                                                  \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return afail((a), NMHD_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+FieldPtrs to_fields(float *const p[8]) {
+  FieldPtrs f;
+  for (int i = 0; i < 8; ++i) f.f[i] = p[i];
+  return f;
+}
+
+FieldPtrs to_fields_c(const float *const p[8]) {
+  FieldPtrs f;
+  for (int i = 0; i < 8; ++i) f.f[i] = const_cast<float *>(p[i]);
+  return f;
+}
+
+FluxSet flux_set(nmhd_adj *a) {
+  FluxSet F;
+  for (int d = 0; d < 3; ++d)
+    for (int k = 0; k < 7; ++k) F.dir[d].f[k] = a->flux[d * 7 + k];
+  return F;
+}
+
+// W_out = S(W_in; dt) with the production stage kernels.
+void enqueue_forward(nmhd_adj *a, FieldPtrs Win, FieldPtrs Wstar, FieldPtrs Wout, float dt) {
+  const FluxSet F = flux_set(a);
+  launch_adj_set_dt(a->stream, a->st, dt);
+  if (a->desc.integrator == NMHD_EULER1) {
+    launch_faces(a->stream, a->box, Win, F, a->st, H_FULL);
+    launch_update_tiled(a->stream, a->box, Win, F, Win, Wout, a->st, H_FULL, false, false);
+    return;
+  }
+  launch_faces(a->stream, a->box, Win, F, a->st, H_HALF);
+  launch_update_tiled(a->stream, a->box, Win, F, Win, Wstar, a->st, H_HALF, false, false);
+  launch_faces(a->stream, a->box, Wstar, F, a->st, H_FULL);
+  launch_update_tiled(a->stream, a->box, Wstar, F, Win, Wout, a->st, H_FULL, false, false);
+}
+
+AdjStageArgs stage_args(nmhd_adj *a) {
+  AdjStageArgs A;
+  std::memset(&A, 0, sizeof(A));
+  float **w = a->work;
+  for (int i = 0; i < 5; ++i) A.gU[i] = *w++;
+  for (int i = 0; i < 3; ++i) A.gB[i] = *w++;
+  for (int i = 0; i < 3; ++i) A.lE[i] = *w++;
+  for (int i = 0; i < 3; ++i) A.lbb[i] = *w++;
+  for (int i = 0; i < 3; ++i) A.abn[i] = *w++;
+  for (int i = 0; i < 14; ++i) A.face[i] = *w++;
+  return A;
+}
+
+void free_adj(nmhd_adj *a) {
+  float **groups[] = {a->wstar, a->wnext, a->lstar, a->scratch_out};
+  for (float **g : groups)
+    for (int i = 0; i < 8; ++i)
+      if (g[i]) (void)hipFree(g[i]), g[i] = nullptr;
+  for (auto &p : a->flux)
+    if (p) (void)hipFree(p), p = nullptr;
+  for (auto &p : a->work)
+    if (p) (void)hipFree(p), p = nullptr;
+  if (a->st) (void)hipFree(a->st), a->st = nullptr;
+  if (a->own) (void)hipStreamDestroy(a->own), a->own = nullptr;
+}
+
+bool ptrs_ok(const float *const p[8]) {
+  if (!p) return false;
+  for (int i = 0; i < 8; ++i)
+    if (!p[i]) return false;
+  return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *nmhd_adj_last_error(const nmhd_adj *a) {
+  return a ? a->err.c_str() : g_adj_create_error.c_str();
+}
+
+int nmhd_adj_create(const nmhd_desc *desc, nmhd_adj **out) {
+  if (!desc || !out) return afail(nullptr, NMHD_EINVAL, "null argument");
+  *out = nullptr;
+  std::string why;
+  if (!desc_valid(desc, why)) return afail(nullptr, NMHD_EINVAL, why);
+  if (desc->margin[0] || desc->margin[1] || desc->margin[2] ||
+      desc->source != NMHD_SOURCE_NONE || desc->shear_y)
+    return afail(nullptr, NMHD_EINVAL,
+                 "the adjoint step is for periodic boxes (margin 0, source NONE, shear_y 0)");
+  nmhd_adj *a = new nmhd_adj();
+  a->desc = *desc;
+  derive_box(desc, a->box, a->geo);
+  auto bail = [&](int code, const std::string &msg) {
+    free_adj(a);
+    delete a;
+    return afail(nullptr, code, msg);
+  };
+  if (hipStreamCreateWithFlags(&a->own, hipStreamNonBlocking) != hipSuccess)
+    return bail(NMHD_EHIP, "hipStreamCreate failed");
+  a->stream = a->own;
+  const size_t bytes = sizeof(float) * (size_t)a->box.ncell;
+  float **groups[] = {a->wstar, a->wnext, a->lstar, a->scratch_out};
+  for (float **g : groups)
+    for (int i = 0; i < 8; ++i)
+      if (hipMalloc(&g[i], bytes) != hipSuccess) return bail(NMHD_ENOMEM, "allocation failed");
+  for (auto &p : a->flux)
+    if (hipMalloc(&p, bytes) != hipSuccess) return bail(NMHD_ENOMEM, "allocation failed");
+  for (auto &p : a->work)
+    if (hipMalloc(&p, bytes) != hipSuccess) return bail(NMHD_ENOMEM, "allocation failed");
+  if (hipMalloc(&a->st, sizeof(StepState)) != hipSuccess)
+    return bail(NMHD_ENOMEM, "allocation failed");
+  StepState h;
+  std::memset(&h, 0, sizeof(h));
+  if (hipMemcpy(a->st, &h, sizeof(h), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(NMHD_EHIP, "step record init failed");
+  *out = a;
+  return NMHD_OK;
+}
+
+int nmhd_adj_destroy(nmhd_adj *a) {
+  if (!a) return NMHD_OK;
+  if (a->stream) (void)hipStreamSynchronize(a->stream);
+  free_adj(a);
+  delete a;
+  return NMHD_OK;
+}
+
+int nmhd_adj_get_geometry(const nmhd_adj *a, nmhd_geometry *geo) {
+  if (!a || !geo) return NMHD_EINVAL;
+  *geo = a->geo;
+  return NMHD_OK;
+}
+
+int nmhd_adj_set_stream(nmhd_adj *a, void *stream) {
+  if (!a) return NMHD_EINVAL;
+  AHIP(a, hipStreamSynchronize(a->stream));
+  a->stream = stream ? (hipStream_t)stream : a->own;
+  return NMHD_OK;
+}
+
+int nmhd_adj_synchronize(nmhd_adj *a) {
+  if (!a) return NMHD_EINVAL;
+  AHIP(a, hipGetLastError());
+  AHIP(a, hipStreamSynchronize(a->stream));
+  return NMHD_OK;
+}
+
+int nmhd_adj_cfl_dt(nmhd_adj *a, const float *const w[8], float goal_time, float *dt) {
+  if (!a || !ptrs_ok(w) || !dt) return afail(a, NMHD_EINVAL, "bad argument");
+  StepState h;
+  std::memset(&h, 0, sizeof(h));
+  h.cfl_bits = 0x7f800000u;
+  AHIP(a, hipMemcpyAsync(a->st, &h, sizeof(h), hipMemcpyHostToDevice, a->stream));
+  launch_cfl(a->stream, a->box, to_fields_c(w), goal_time, a->st);
+  AHIP(a, hipMemcpyAsync(&h, a->st, sizeof(h), hipMemcpyDeviceToHost, a->stream));
+  AHIP(a, hipStreamSynchronize(a->stream));
+  std::memcpy(dt, &h.cfl_bits, sizeof(float));
+  return NMHD_OK;
+}
+
+int nmhd_adj_forward(nmhd_adj *a, const float *const w_in[8], float dt, float *const w_out[8]) {
+  if (!a || !ptrs_ok(w_in) || !w_out || !ptrs_ok(w_out)) return afail(a, NMHD_EINVAL, "bad argument");
+  for (int i = 0; i < 8; ++i)
+    if (w_in[i] == w_out[i]) return afail(a, NMHD_EINVAL, "w_out must not alias w_in");
+  enqueue_forward(a, to_fields_c(w_in), to_fields(a->wstar), to_fields(w_out), dt);
+  AHIP(a, hipGetLastError());
+  return NMHD_OK;
+}
+
+int nmhd_adj_reverse(nmhd_adj *a, const float *const w_in[8], float dt,
+                     const float *const lam_out[8], float *const lam_in[8]) {
+  if (!a || !ptrs_ok(w_in) || !ptrs_ok(lam_out) || !lam_in || !ptrs_ok(lam_in))
+    return afail(a, NMHD_EINVAL, "bad argument");
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 8; ++j)
+      if (lam_in[i] == lam_out[j] || lam_in[i] == w_in[j])
+        return afail(a, NMHD_EINVAL, "lam_in must not alias w_in or lam_out");
+  const FieldPtrs Wn = to_fields_c(w_in);
+  const FieldPtrs Ws = to_fields(a->wstar);
+  const FieldPtrs Wnext = to_fields(a->wnext);
+  enqueue_forward(a, Wn, Ws, Wnext, dt);
+  AdjStageArgs A = stage_args(a);
+  if (a->desc.integrator == NMHD_EULER1) {
+    A.Win = Wn, A.Wbase = Wn, A.Wout = Wnext;
+    A.lam_out = to_fields_c(lam_out);
+    A.lam_base = to_fields(lam_in);
+    A.lam_in = to_fields(lam_in);
+    A.h = dt, A.acc_base = false, A.acc_in = true;
+    launch_adj_stage(a->stream, a->box, A);
+  } else {
+    // stage 2: W^{n+1} = prim(cons(W^n) + dt P(W*))
+    A.Win = Ws, A.Wbase = Wn, A.Wout = Wnext;
+    A.lam_out = to_fields_c(lam_out);
+    A.lam_base = to_fields(lam_in);
+    A.lam_in = to_fields(a->lstar);
+    A.h = dt, A.acc_base = false, A.acc_in = false;
+    launch_adj_stage(a->stream, a->box, A);
+    // stage 1: W* = prim(cons(W^n) + dt/2 P(W^n))
+    A.Win = Wn, A.Wbase = Wn, A.Wout = Ws;
+    A.lam_out = to_fields(a->lstar);
+    A.lam_base = to_fields(lam_in);
+    A.lam_in = to_fields(lam_in);
+    A.h = dt / 2, A.acc_base = true, A.acc_in = true;
+    launch_adj_stage(a->stream, a->box, A);
+  }
+  AHIP(a, hipGetLastError());
+  return NMHD_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,424 @@
+// kernels_adjoint.hip -- the hand-written reverse (adjoint) of one RK stage
+// of the periodic step, U_out = cons(W_base) + h P(W_in), W_out = prim(U_out),
+// with dt frozen.  Given lam_out = dL/dW_out it produces dL/dW_base and
+// dL/dW_in.  No reference counterpart: the reference has no derivatives; the
+// forward it reverses is proceed()'s stage (library/mhd_solver.inl:
+// 990-1071 for cons/prim/update, 760-897 for the fluxes and CT, HLLD
+// library/hlld.inl:7-284).
+//
+// Kernels of one stage reverse (all gathers: every thread writes only its
+// own cell, no atomics, results independent of scheduling):
+//   k_adj_prim    adjoint of prim(U) and of cons(W_base): gU (5 per cell),
+//                 gB (3 face values per cell) and lam_base
+//   k_adj_edges   adjoint of the 6 CT edge-EMF families (3 distinct values per
+//                 cell: lamE_x, lamE_y, lamE_z)
+//   k_adj_face<D> per face: adjoint of the 7 fluxes (divergence + upwinded
+//                 EMFs, with the forward's upwind choice recomputed), then the
+//                 HLLD reverse sweep (adjoint_hlld.h) -> adjoints of the 7
+//                 left/right face states and of the normal field
+//   k_adj_gather<D> per cell: reverse of the minmod reconstruction from the 4
+//                 faces whose stencils hold the cell (forward minmod branches
+//                 recomputed) -> lam_in of rho, v, p and of the cell-averaged
+//                 transverse B
+//   k_adj_bfinish face B: cell-average and normal-field adjoints -> lam_in B
+#include "adjoint_hlld.h"
+#include "nmhd_kernels.h"
+
+namespace nmhd {
+
+namespace {
+
+__device__ __forceinline__ long cell_index(const Box &b, int x, int y, int z) {
+  return ((long)z * b.n[1] + y) * b.n[0] + x;
+}
+
+// neighbour of (x, y, z) shifted by s along axis d (periodic)
+__device__ __forceinline__ long nb(const Box &b, int x, int y, int z, int d, int s) {
+  if (d == 0) x = wrap(x + s, b.n[0]);
+  if (d == 1) y = wrap(y + s, b.n[1]);
+  if (d == 2) z = wrap(z + s, b.n[2]);
+  return cell_index(b, x, y, z);
+}
+
+__device__ __forceinline__ bool cell_of(const Box &b, int &x, int &y, int &z, long &a) {
+  a = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= b.ncell) return false;
+  x = (int)(a % b.n[0]);
+  y = (int)((a / b.n[0]) % b.n[1]);
+  z = (int)(a / b.n[0] / b.n[1]);
+  return true;
+}
+
+// weights of the two candidates of the upwind choice s > 0 ? f0 : s < 0 ? f1 : avg
+__device__ __forceinline__ float w_lo(float s) { return s > 0.0f ? 1.0f : (s < 0.0f ? 0.0f : 0.5f); }
+__device__ __forceinline__ float w_hi(float s) { return s > 0.0f ? 0.0f : (s < 0.0f ? 1.0f : 0.5f); }
+
+// The forward's sum of four cell velocities around an edge (same order as the
+// stage kernels, so that the sign decision is the same).  The edge lies on
+// the lower p-face and lower q-face of cell (x,y,z) (p < q axes); comp is
+// the velocity component summed.
+__device__ __forceinline__ float edge_sum(const Box &b, const float *v, int x, int y, int z,
+                                          int p, int q, int comp) {
+  const long c = cell_index(b, x, y, z);
+  const long cp = nb(b, x, y, z, p, -1);
+  const long cq = nb(b, x, y, z, q, -1);
+  int xp = x, yp = y, zp = z;
+  if (p == 0) xp = wrap(x - 1, b.n[0]);
+  if (p == 1) yp = wrap(y - 1, b.n[1]);
+  const long cpq = nb(b, xp, yp, zp, q, -1);
+  // winds summed along the first edge axis first when comp is the second
+  // edge axis' component, as the stage kernels do
+  if (comp == q) return v[c] + v[cq] + v[cp] + v[cpq];
+  return v[c] + v[cp] + v[cq] + v[cpq];
+}
+
+// ------------------------------------------------------------- k_adj_prim
+
+template <bool AccBase>
+__global__ void __launch_bounds__(256) k_adj_prim(Box b, FieldPtrs Wout, FieldPtrs Wbase,
+                                                  FieldPtrs lam, float *gU0, float *gU1,
+                                                  float *gU2, float *gU3, float *gU4, float *gB0,
+                                                  float *gB1, float *gB2, FieldPtrs lam_base) {
+  int x, y, z;
+  long a;
+  if (!cell_of(b, x, y, z, a)) return;
+  const float gm1 = b.gm1, ig = b.inv_gm1;
+  const float rho = Wout.f[0][a];
+  const float v[3] = {Wout.f[1][a], Wout.f[2][a], Wout.f[3][a]};
+  const float lp = lam.f[4][a];
+  const float irho = 1.0f / rho;
+  const float gE = gm1 * lp;
+  float gm[3];
+  float grho = lam.f[0][a] + gm1 * lp * 0.5f * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  for (int d = 0; d < 3; ++d) {
+    gm[d] = lam.f[1 + d][a] * irho - gm1 * lp * v[d];
+    grho -= lam.f[1 + d][a] * v[d] * irho;
+  }
+  // face B_d (lower d-face): its own adjoint, and through the pressure of
+  // the two cells that share it
+  float gB[3];
+  for (int d = 0; d < 3; ++d) {
+    const long m = nb(b, x, y, z, d, -1);
+    gB[d] = lam.f[5 + d][a] - gm1 * 0.5f * Wout.f[5 + d][a] * (lp + lam.f[4][m]);
+  }
+  gU0[a] = grho;
+  gU1[a] = gm[0];
+  gU2[a] = gm[1];
+  gU3[a] = gm[2];
+  gU4[a] = gE;
+  gB0[a] = gB[0];
+  gB1[a] = gB[1];
+  gB2[a] = gB[2];
+  // cons(W_base): rho, m = rho v, E = p/(g-1) + rho v^2/2 + (B_lo^2 + B_hi^2)/4
+  const float rho0 = Wbase.f[0][a];
+  const float v0[3] = {Wbase.f[1][a], Wbase.f[2][a], Wbase.f[3][a]};
+  float out[8];
+  out[0] = grho + 0.5f * gE * (v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2]);
+  for (int d = 0; d < 3; ++d) {
+    out[0] += gm[d] * v0[d];
+    out[1 + d] = gm[d] * rho0 + gE * rho0 * v0[d];
+  }
+  out[4] = gE * ig;
+  for (int d = 0; d < 3; ++d) {
+    const long m = nb(b, x, y, z, d, -1);
+    out[5 + d] = gB[d] + 0.5f * Wbase.f[5 + d][a] * (gE + gm1 * lam.f[4][m]);
+  }
+  for (int f = 0; f < 8; ++f) lam_base.f[f][a] = AccBase ? lam_base.f[f][a] + out[f] : out[f];
+}
+
+// ------------------------------------------------------------ k_adj_edges
+//
+// lamE_z = adjoint of the z-edge EMF from the x-sweep (= minus that of the
+// y-sweep's), at the edge on the lower x- and y-faces of the cell; likewise
+// lamE_y (x-sweep / z-sweep pair on the lower x- and z-faces) and lamE_x
+// (y-sweep / z-sweep pair on the lower y- and z-faces).  gB is scaled by h.
+__global__ void __launch_bounds__(256) k_adj_edges(Box b, const float *gB0, const float *gB1,
+                                                   const float *gB2, float h, float *lEx,
+                                                   float *lEy, float *lEz) {
+  int x, y, z;
+  long a;
+  if (!cell_of(b, x, y, z, a)) return;
+  const float idx = b.inv_d[0], idy = b.inv_d[1], idz = b.inv_d[2];
+  const long xm = nb(b, x, y, z, 0, -1), ym = nb(b, x, y, z, 1, -1), zm = nb(b, x, y, z, 2, -1);
+  lEz[a] = h * (0.5f * idy * (gB0[ym] - gB0[a]) + 0.5f * idx * (gB1[a] - gB1[xm]));
+  lEy[a] = h * (0.5f * idz * (gB0[zm] - gB0[a]) + 0.5f * idx * (gB2[a] - gB2[xm]));
+  lEx[a] = h * (0.5f * idz * (gB1[zm] - gB1[a]) + 0.5f * idy * (gB2[a] - gB2[ym]));
+}
+
+// ------------------------------------------------------------- k_adj_face
+//
+// Face = lower D-face of cell (x,y,z).  Flux components in the forward's
+// order (rho, m_D, m_D+1, m_D+2, E, B_D+1, B_D+2).
+struct AdjFaceOut {
+  float *aL[7];
+  float *aR[7];
+  float *abn;
+};
+
+template <int D>
+__global__ void __launch_bounds__(256) k_adj_face(Box b, FieldPtrs W, const float *gU0,
+                                                  const float *gU1, const float *gU2,
+                                                  const float *gU3, const float *gU4,
+                                                  const float *lEx, const float *lEy,
+                                                  const float *lEz, float h, AdjFaceOut out) {
+  int x, y, z;
+  long a;
+  if (!cell_of(b, x, y, z, a)) return;
+  constexpr int D1 = (D + 1) % 3, D2 = (D + 2) % 3;
+  const float *gU[5] = {gU0, gU1, gU2, gU3, gU4};
+  const long m1 = nb(b, x, y, z, D, -1);
+  const float id = b.inv_d[D];
+  adj::HOut<float> aF;
+  aF.rho = h * (gU[0][a] - gU[0][m1]) * id;
+  aF.mn = h * (gU[1 + D][a] - gU[1 + D][m1]) * id;
+  aF.mt1 = h * (gU[1 + D1][a] - gU[1 + D1][m1]) * id;
+  aF.mt2 = h * (gU[1 + D2][a] - gU[1 + D2][m1]) * id;
+  aF.e = h * (gU[4][a] - gU[4][m1]) * id;
+  // B fluxes through the upwinded edge EMFs.  For the x-sweep: F(B_y) feeds
+  // the z-edges on y-faces j (as the upper candidate) and j+1 (lower one),
+  // upwinded by the y-velocity; F(B_z) the y-edges on z-faces k and k+1.
+  const float *v[3] = {W.f[1], W.f[2], W.f[3]};
+  {
+    // component bt1 = B_D1 flux, used by edges along D2 on D1-faces
+    // component bt2 = B_D2 flux, used by edges along D1 on D2-faces
+    auto edge_adj = [&](int q, int xx, int yy, int zz) -> float {
+      // lam of the edge on the lower D-face and lower q-face of cell (xx,yy,zz),
+      // with the sign of the D-sweep's member of the pair
+      const long e = cell_index(b, xx, yy, zz);
+      const int other = 3 - D - q;  // the edge's axis
+      const float *arr = other == 0 ? lEx : (other == 1 ? lEy : lEz);
+      // the x-sweep (or, for x-edges, the y-sweep) member carries +lam,
+      // the other sweep's member -lam
+      const int first = D < q ? D : q;
+      return D == first ? arr[e] : -arr[e];
+    };
+    auto sum_at = [&](int q, int xx, int yy, int zz) -> float {
+      // upwind sum of the q-velocity around the edge on the lower D- and
+      // q-faces of cell (xx,yy,zz), in the forward's order
+      const int p = D < q ? D : q, r = D < q ? q : D;
+      return edge_sum(b, v[q], xx, yy, zz, p, r, q);
+    };
+    int xs[3] = {x, y, z};
+    // bt1: flux of B_D1, edges on D1-faces j (this face's own row) and j+1
+    {
+      const int q = D1;
+      int up[3] = {xs[0], xs[1], xs[2]};
+      up[q] = wrap(up[q] + 1, b.n[q]);
+      const float s0 = sum_at(q, xs[0], xs[1], xs[2]);
+      const float s1 = sum_at(q, up[0], up[1], up[2]);
+      aF.bt1 = edge_adj(q, xs[0], xs[1], xs[2]) * w_hi(s0) +
+               edge_adj(q, up[0], up[1], up[2]) * w_lo(s1);
+    }
+    {
+      const int q = D2;
+      int up[3] = {xs[0], xs[1], xs[2]};
+      up[q] = wrap(up[q] + 1, b.n[q]);
+      const float s0 = sum_at(q, xs[0], xs[1], xs[2]);
+      const float s1 = sum_at(q, up[0], up[1], up[2]);
+      aF.bt2 = edge_adj(q, xs[0], xs[1], xs[2]) * w_hi(s0) +
+               edge_adj(q, up[0], up[1], up[2]) * w_lo(s1);
+    }
+  }
+  // recompute the face states (same reconstruction as the forward)
+  const long c0 = nb(b, x, y, z, D, -2), c1 = m1, c2 = a, c3 = nb(b, x, y, z, D, 1);
+  auto bavg = [&](int t, long c) -> float {
+    int cx = (int)(c % b.n[0]), cy = (int)((c / b.n[0]) % b.n[1]), cz = (int)(c / b.n[0] / b.n[1]);
+    return 0.5f * (W.f[5 + t][c] + W.f[5 + t][nb(b, cx, cy, cz, t, 1)]);
+  };
+  float q[7][4];
+  const long cs[4] = {c0, c1, c2, c3};
+  for (int s = 0; s < 4; ++s) {
+    q[0][s] = W.f[0][cs[s]];
+    q[1][s] = W.f[1 + D][cs[s]];
+    q[2][s] = W.f[1 + D1][cs[s]];
+    q[3][s] = W.f[1 + D2][cs[s]];
+    q[4][s] = W.f[4][cs[s]];
+    q[5][s] = bavg(D1, cs[s]);
+    q[6][s] = bavg(D2, cs[s]);
+  }
+  float Lq[7], Rq[7];
+  for (int k = 0; k < 7; ++k) {
+    if (b.recon == 1) {
+      Lq[k] = q[k][1];
+      Rq[k] = q[k][2];
+    } else {
+      recon_minbee2(q[k][0], q[k][1], q[k][2], q[k][3], Lq[k], Rq[k]);
+    }
+  }
+  adj::HIn<float> in;
+  in.bx = W.f[5 + D][a];
+  in.rL = Lq[0], in.rR = Rq[0], in.unL = Lq[1], in.unR = Rq[1], in.u1L = Lq[2], in.u1R = Rq[2];
+  in.u2L = Lq[3], in.u2R = Rq[3], in.pL = Lq[4], in.pR = Rq[4], in.b1L = Lq[5], in.b1R = Rq[5];
+  in.b2L = Lq[6], in.b2R = Rq[6];
+  adj::HTape<float> tape;
+  adj::HOut<float> F;
+  adj::hlld_forward(b.gamma, b.inv_gm1, in, tape, F);
+  adj::HIn<float> g;
+  adj::hlld_reverse(b.gamma, b.inv_gm1, in, tape, aF, g);
+  const float gl[7] = {g.rL, g.unL, g.u1L, g.u2L, g.pL, g.b1L, g.b2L};
+  const float gr[7] = {g.rR, g.unR, g.u1R, g.u2R, g.pR, g.b1R, g.b2R};
+  for (int k = 0; k < 7; ++k) {
+    out.aL[k][a] = gl[k];
+    out.aR[k][a] = gr[k];
+  }
+  out.abn[a] = g.bx;
+}
+
+// ------------------------------------------------------------ k_adj_gather
+//
+// Cell c, variable x along D: L(face c+1) = x_c + s_c/2, R(face c) =
+// x_c - s_c/2 with the limited slope s_c = minmod(x_c - x_{c-1}, x_{c+1} - x_c).
+__device__ __forceinline__ void mm_grad(float a, float b, float &da, float &db) {
+  const bool nz = !(a * b <= 0.0f);
+  const bool pick_a = fabsf(a) < fabsf(b);
+  da = (nz && pick_a) ? 1.0f : 0.0f;
+  db = (nz && !pick_a) ? 1.0f : 0.0f;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256) k_adj_gather(Box b, FieldPtrs W, AdjFaceOut fa,
+                                                    FieldPtrs lam_in, int first,
+                                                    float *lbb0, float *lbb1, float *lbb2) {
+  int x, y, z;
+  long a;
+  if (!cell_of(b, x, y, z, a)) return;
+  constexpr int D1 = (D + 1) % 3, D2 = (D + 2) % 3;
+  // cells c-2..c+2 and faces c-1..c+2 along D
+  long cs[5];
+  for (int s = 0; s < 5; ++s) cs[s] = nb(b, x, y, z, D, s - 2);
+  auto bavg = [&](int t, long c) -> float {
+    int cx = (int)(c % b.n[0]), cy = (int)((c / b.n[0]) % b.n[1]), cz = (int)(c / b.n[0] / b.n[1]);
+    return 0.5f * (W.f[5 + t][c] + W.f[5 + t][nb(b, cx, cy, cz, t, 1)]);
+  };
+  float res[7];
+  for (int k = 0; k < 7; ++k) {
+    float xv[5];
+    for (int s = 0; s < 5; ++s) {
+      const long c = cs[s];
+      xv[s] = k == 0 ? W.f[0][c]
+            : k == 1 ? W.f[1 + D][c]
+            : k == 2 ? W.f[1 + D1][c]
+            : k == 3 ? W.f[1 + D2][c]
+            : k == 4 ? W.f[4][c]
+            : k == 5 ? bavg(D1, c)
+                     : bavg(D2, c);
+    }
+    // face f of cell cs[s] is its lower face; aL(face c+1) = fa.aL[k][cs[3]]
+    const float aL_c = fa.aL[k][cs[2]], aL_c1 = fa.aL[k][cs[3]], aL_c2 = fa.aL[k][cs[4]];
+    const float aR_cm1 = fa.aR[k][cs[1]], aR_c = fa.aR[k][cs[2]], aR_c1 = fa.aR[k][cs[3]];
+    float r = aL_c1 + aR_c;
+    if (b.recon != 1) {
+      // slope adjoints of cells c-1, c, c+1
+      const float as_m = 0.5f * (aL_c - aR_cm1);
+      const float as_0 = 0.5f * (aL_c1 - aR_c);
+      const float as_p = 0.5f * (aL_c2 - aR_c1);
+      float da_m, db_m, da_0, db_0, da_p, db_p;
+      mm_grad(xv[1] - xv[0], xv[2] - xv[1], da_m, db_m);
+      mm_grad(xv[2] - xv[1], xv[3] - xv[2], da_0, db_0);
+      mm_grad(xv[3] - xv[2], xv[4] - xv[3], da_p, db_p);
+      r += as_0 * (da_0 - db_0) - as_p * da_p + as_m * db_m;
+    }
+    res[k] = r;
+  }
+  // rho, v, p into lam_in (velocity components permuted back)
+  const int vf[3] = {1 + D, 1 + D1, 1 + D2};
+  if (first) {
+    lam_in.f[0][a] = res[0];
+    for (int i = 0; i < 3; ++i) lam_in.f[vf[i]][a] = res[1 + i];
+    lam_in.f[4][a] = res[4];
+  } else {
+    lam_in.f[0][a] += res[0];
+    for (int i = 0; i < 3; ++i) lam_in.f[vf[i]][a] += res[1 + i];
+    lam_in.f[4][a] += res[4];
+  }
+  // cell-averaged transverse fields
+  float *lbb[3] = {lbb0, lbb1, lbb2};
+  lbb[D1][a] += res[5];
+  lbb[D2][a] += res[6];
+}
+
+// Zero the cell-average accumulators (a fill kernel, capturable in graphs).
+__global__ void k_adj_zero3(long n, float *p0, float *p1, float *p2) {
+  const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= n) return;
+  p0[a] = 0.0f;
+  p1[a] = 0.0f;
+  p2[a] = 0.0f;
+}
+
+// Face B_t of lam_in: half of each adjacent cell's cell-average adjoint plus
+// the normal-field adjoint of the t-sweep's face.
+__global__ void __launch_bounds__(256) k_adj_bfinish(Box b, const float *lbb0, const float *lbb1,
+                                                     const float *lbb2, const float *abn0,
+                                                     const float *abn1, const float *abn2,
+                                                     FieldPtrs lam_in, int accumulate) {
+  int x, y, z;
+  long a;
+  if (!cell_of(b, x, y, z, a)) return;
+  const float *lbb[3] = {lbb0, lbb1, lbb2};
+  const float *abn[3] = {abn0, abn1, abn2};
+  for (int t = 0; t < 3; ++t) {
+    const long m = nb(b, x, y, z, t, -1);
+    const float v = 0.5f * (lbb[t][a] + lbb[t][m]) + abn[t][a];
+    lam_in.f[5 + t][a] = accumulate ? lam_in.f[5 + t][a] + v : v;
+  }
+}
+
+// Step record of a stage with a frozen dt.
+__global__ void k_adj_set_dt(StepState *st, float dt) {
+  st->dt = dt;
+  st->h = dt / 2;
+  st->h_diag = dt;
+  st->active = 1;
+  st->finished = 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+
+void launch_adj_set_dt(hipStream_t s, StepState *st, float dt) {
+  hipLaunchKernelGGL(k_adj_set_dt, dim3(1), dim3(1), 0, s, st, dt);
+}
+
+void launch_adj_stage(hipStream_t s, const Box &b, const AdjStageArgs &A) {
+  const dim3 g(grid_for(b.ncell)), blk(kBlock);
+  if (A.acc_base)
+    hipLaunchKernelGGL(k_adj_prim<true>, g, blk, 0, s, b, A.Wout, A.Wbase, A.lam_out, A.gU[0],
+                       A.gU[1], A.gU[2], A.gU[3], A.gU[4], A.gB[0], A.gB[1], A.gB[2], A.lam_base);
+  else
+    hipLaunchKernelGGL(k_adj_prim<false>, g, blk, 0, s, b, A.Wout, A.Wbase, A.lam_out, A.gU[0],
+                       A.gU[1], A.gU[2], A.gU[3], A.gU[4], A.gB[0], A.gB[1], A.gB[2], A.lam_base);
+  hipLaunchKernelGGL(k_adj_edges, g, blk, 0, s, b, A.gB[0], A.gB[1], A.gB[2], A.h, A.lE[0],
+                     A.lE[1], A.lE[2]);
+  hipLaunchKernelGGL(k_adj_zero3, g, blk, 0, s, b.ncell, A.lbb[0], A.lbb[1], A.lbb[2]);
+  for (int d = 0; d < 3; ++d) {
+    AdjFaceOut fo;
+    for (int k = 0; k < 7; ++k) {
+      fo.aL[k] = A.face[k];
+      fo.aR[k] = A.face[7 + k];
+    }
+    fo.abn = A.abn[d];
+    const int first = (d == 0 && !A.acc_in) ? 1 : 0;
+    if (d == 0) {
+      hipLaunchKernelGGL(k_adj_face<0>, g, blk, 0, s, b, A.Win, A.gU[0], A.gU[1], A.gU[2],
+                         A.gU[3], A.gU[4], A.lE[0], A.lE[1], A.lE[2], A.h, fo);
+      hipLaunchKernelGGL(k_adj_gather<0>, g, blk, 0, s, b, A.Win, fo, A.lam_in, first, A.lbb[0],
+                         A.lbb[1], A.lbb[2]);
+    } else if (d == 1) {
+      hipLaunchKernelGGL(k_adj_face<1>, g, blk, 0, s, b, A.Win, A.gU[0], A.gU[1], A.gU[2],
+                         A.gU[3], A.gU[4], A.lE[0], A.lE[1], A.lE[2], A.h, fo);
+      hipLaunchKernelGGL(k_adj_gather<1>, g, blk, 0, s, b, A.Win, fo, A.lam_in, first, A.lbb[0],
+                         A.lbb[1], A.lbb[2]);
+    } else {
+      hipLaunchKernelGGL(k_adj_face<2>, g, blk, 0, s, b, A.Win, A.gU[0], A.gU[1], A.gU[2],
+                         A.gU[3], A.gU[4], A.lE[0], A.lE[1], A.lE[2], A.h, fo);
+      hipLaunchKernelGGL(k_adj_gather<2>, g, blk, 0, s, b, A.Win, fo, A.lam_in, first, A.lbb[0],
+                         A.lbb[1], A.lbb[2]);
+    }
+  }
+  hipLaunchKernelGGL(k_adj_bfinish, g, blk, 0, s, b, A.lbb[0], A.lbb[1], A.lbb[2], A.abn[0],
+                     A.abn[1], A.abn[2], A.lam_in, A.acc_in ? 1 : 0);
+}
+
+}  // namespace nmhd

@@ -1,0 +1,191 @@
+// kernels_batch.hip -- many independent periodic boxes of uneven sizes
+// advanced together.  Each RK stage is two persistent launches (face fluxes,
+// then the update) whose workgroups walk a work table of tiles from every
+// box; a tile's box record gives its sizes, its per-box step (own dt) and the
+// offsets of its fields in the batch's pools.  The tile functions are the
+// single-box path's (tiled_tiles.h), so a box advanced in a batch is
+// bit-identical to the same box advanced alone.  The bookkeeping kernels run
+// one thread per box on the per-box step records.
+//
+// The reference advances one compile-time-sized box per process and has no
+// counterpart of this path; its per-box arithmetic is that of proceed()
+// (library/mhd_solver.inl:1086-1204).
+#include "tiled_tiles.h"
+
+namespace nmhd {
+
+namespace {
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int kFacesLds = cmax(cmax(tiled::FacesShape<64, 4>::kLds, tiled::FacesShape<32, 8>::kLds),
+                               tiled::FacesShape<16, 16>::kLds);
+constexpr int kUpdateLds =
+    cmax(cmax(tiled::UpdateShape<64, 4>::kLds, tiled::UpdateShape<32, 8>::kLds),
+         tiled::UpdateShape<16, 16>::kLds);
+
+__device__ __forceinline__ void item_tile(const WorkItem &w, int &x0, int &y0, int &z0,
+                                          int &zend) {
+  x0 = w.xy & 0xffff;
+  y0 = (w.xy >> 16) & 0xffff;
+  z0 = w.z & 0xffff;
+  zend = (w.z >> 16) & 0xffff;
+}
+
+__global__ void __launch_bounds__(tiled::kTileThreads)
+    k_batch_faces(const BatchBox *boxes, const WorkItem *items, int nitems, float *state,
+                  float *flux, const StepState *st, int hsel) {
+  __shared__ float lds[kFacesLds];
+  const int G = gridDim.x;
+  for (int it = xcd_logical_block(blockIdx.x, G); it < nitems; it += G) {
+    const WorkItem w = items[it];
+    const StepState *s = st + w.box;
+    if (!s->active) continue;
+    const BatchBox bb = boxes[w.box];
+    const FieldPtrs W = box_fields(state, bb);
+    const FluxSet F = box_fluxes(flux, bb);
+    const float h = tiled::stage_h(s, hsel);
+    int x0, y0, z0, zend;
+    item_tile(w, x0, y0, z0, zend);
+    switch (w.variant) {
+      case TV_64x4:
+        tiled::faces_tile<64, 4>(bb.b, W, F, h, x0, y0, z0, zend, lds);
+        break;
+      case TV_32x8:
+        tiled::faces_tile<32, 8>(bb.b, W, F, h, x0, y0, z0, zend, lds);
+        break;
+      default:
+        tiled::faces_tile<16, 16>(bb.b, W, F, h, x0, y0, z0, zend, lds);
+        break;
+    }
+  }
+}
+
+template <bool Cfl>
+__global__ void __launch_bounds__(tiled::kTileThreads)
+    k_batch_update(const BatchBox *boxes, const WorkItem *items, int nitems, float *state_in,
+                   float *flux, float *state_base, float *state_out, StepState *st, int hsel) {
+  __shared__ float lds[kUpdateLds];
+  __shared__ float red[4];
+  const int G = gridDim.x;
+  for (int it = xcd_logical_block(blockIdx.x, G); it < nitems; it += G) {
+    const WorkItem w = items[it];
+    StepState *s = st + w.box;
+    if (!s->active) continue;
+    const BatchBox bb = boxes[w.box];
+    const FieldPtrs Win = box_fields(state_in, bb);
+    const FieldPtrs Wb = box_fields(state_base, bb);
+    const FieldPtrs Wo = box_fields(state_out, bb);
+    const FluxSet F = box_fluxes(flux, bb);
+    const float h = tiled::stage_h(s, hsel);
+    int x0, y0, z0, zend;
+    item_tile(w, x0, y0, z0, zend);
+    float cfl_t = 3.0e38f;
+    switch (w.variant) {
+      case TV_64x4:
+        tiled::update_tile<64, 4, false, Cfl>(bb.b, Win, F, Wb, Wo, h, x0, y0, z0, zend, lds,
+                                              cfl_t);
+        break;
+      case TV_32x8:
+        tiled::update_tile<32, 8, false, Cfl>(bb.b, Win, F, Wb, Wo, h, x0, y0, z0, zend, lds,
+                                              cfl_t);
+        break;
+      default:
+        tiled::update_tile<16, 16, false, Cfl>(bb.b, Win, F, Wb, Wo, h, x0, y0, z0, zend, lds,
+                                               cfl_t);
+        break;
+    }
+    if (Cfl) tiled::block_min_into(cfl_t, &s->cfl_next_bits, red);
+  }
+}
+
+__global__ void k_batch_begin(StepState *st, int nbox, int full) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbox) step_begin(st[i], full);
+}
+
+// goal is read from device memory so that a captured step serves any goal.
+__global__ void k_batch_dt(StepState *st, int nbox, const float *goal) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbox) step_dt(st[i], *goal);
+}
+
+__global__ void k_batch_commit(StepState *st, int nbox) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nbox) step_commit(st[i]);
+}
+
+// Full CFL pass: blockIdx.y is the box, blocks stride over its cells.
+__global__ void __launch_bounds__(256) k_batch_cfl(const BatchBox *boxes, float *state,
+                                                   float goal, StepState *st) {
+  const int box = blockIdx.y;
+  if (st[box].finished) return;
+  __shared__ float red[4];
+  const BatchBox bb = boxes[box];
+  const FieldPtrs W = box_fields(state, bb);
+  const Box &b = bb.b;
+  float t = goal;
+  for (long a = (long)blockIdx.x * blockDim.x + threadIdx.x; a < b.ncell;
+       a += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(a % b.n[0]);
+    const int y = (int)((a / b.n[0]) % b.n[1]);
+    const int z = (int)(a / b.n[0] / b.n[1]);
+    t = fminf(t, cell_cfl(b, W, a, x, y, z));
+  }
+  const float m = tiled::wave_min64(t);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float r = fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
+    atomicMin(&st[box].cfl_bits, __float_as_uint(r));
+  }
+}
+
+// Carry the state of boxes whose step is inactive (goal reached) into the
+// next rotation slot.
+__global__ void k_batch_copy(const BatchBox *boxes, float *src, float *dst, const StepState *st) {
+  const int box = blockIdx.y;
+  if (st[box].active) return;
+  const BatchBox bb = boxes[box];
+  const long n = 8 * bb.b.ncell;
+  for (long a = (long)blockIdx.x * blockDim.x + threadIdx.x; a < n;
+       a += (long)gridDim.x * blockDim.x)
+    dst[bb.off8 + a] = src[bb.off8 + a];
+}
+
+}  // namespace
+
+void launch_batch_begin(hipStream_t s, StepState *st, int nbox, bool full) {
+  hipLaunchKernelGGL(k_batch_begin, dim3(grid_for(nbox, 64)), dim3(64), 0, s, st, nbox,
+                     (int)full);
+}
+void launch_batch_cfl(hipStream_t s, const BatchBox *boxes, int nbox, float *state, float goal,
+                      StepState *st) {
+  hipLaunchKernelGGL(k_batch_cfl, dim3(64, nbox), dim3(256), 0, s, boxes, state, goal, st);
+}
+void launch_batch_dt(hipStream_t s, StepState *st, int nbox, const float *goal) {
+  hipLaunchKernelGGL(k_batch_dt, dim3(grid_for(nbox, 64)), dim3(64), 0, s, st, nbox, goal);
+}
+void launch_batch_commit(hipStream_t s, StepState *st, int nbox) {
+  hipLaunchKernelGGL(k_batch_commit, dim3(grid_for(nbox, 64)), dim3(64), 0, s, st, nbox);
+}
+void launch_batch_copy(hipStream_t s, const BatchBox *boxes, int nbox, float *src, float *dst,
+                       const StepState *st) {
+  hipLaunchKernelGGL(k_batch_copy, dim3(32, nbox), dim3(256), 0, s, boxes, src, dst, st);
+}
+void launch_batch_faces(hipStream_t s, int grid, const BatchBox *boxes, const WorkItem *items,
+                        int nitems, float *state, float *flux, const StepState *st, int hsel) {
+  hipLaunchKernelGGL(k_batch_faces, dim3(grid), dim3(tiled::kTileThreads), 0, s, boxes, items,
+                     nitems, state, flux, st, hsel);
+}
+void launch_batch_update(hipStream_t s, int grid, const BatchBox *boxes, const WorkItem *items,
+                         int nitems, float *state_in, float *flux, float *state_base,
+                         float *state_out, StepState *st, int hsel, bool cfl) {
+  if (cfl)
+    hipLaunchKernelGGL(k_batch_update<true>, dim3(grid), dim3(tiled::kTileThreads), 0, s, boxes,
+                       items, nitems, state_in, flux, state_base, state_out, st, hsel);
+  else
+    hipLaunchKernelGGL(k_batch_update<false>, dim3(grid), dim3(tiled::kTileThreads), 0, s, boxes,
+                       items, nitems, state_in, flux, state_base, state_out, st, hsel);
+}
+
+}  // namespace nmhd

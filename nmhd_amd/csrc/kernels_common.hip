@@ -14,13 +14,7 @@
 
 namespace nmhd {
 
-// Start a step.  full: the CFL minimum is rebuilt over every cell by k_cfl;
-// otherwise it starts from the stage-2 epilogue's minimum of the current state
-// and k_cfl_ghosts adds the cells the ghost fill rewrote.
-__global__ void k_begin_step(StepState *st, int full) {
-  st->cfl_bits = full ? 0x7f800000u : st->cfl_next_bits;
-  st->cfl_next_bits = 0x7f800000u;  // +inf
-}
+__global__ void k_begin_step(StepState *st, int full) { step_begin(*st, full); }
 
 // relative_displacement (INL:288-295) with D0 = 0 and
 // relvel = -q*Omega*kLocalExtentY (application/kepler_disk_demo/
@@ -112,7 +106,6 @@ __device__ __forceinline__ float wave_min(float v) {
 // shuffle min, an LDS block min and one atomic min per block on the float
 // bits (valid: all candidates are >= 0).
 __global__ void __launch_bounds__(256) k_cfl(Box b, FieldPtrs W, float goal, StepState *st) {
-#pragma clang fp contract(off)
   if (st->finished) return;
   __shared__ float red[4];
   const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -121,16 +114,7 @@ __global__ void __launch_bounds__(256) k_cfl(Box b, FieldPtrs W, float goal, Ste
     const int x = (int)(a % b.n[0]);
     const int y = (int)((a / b.n[0]) % b.n[1]);
     const int z = (int)(a / b.n[0] / b.n[1]);
-    const float bxl = W.f[5][a], bxh = W.f[5][addr(b, wrap(x + 1, b.n[0]), y, z)];
-    const float byl = W.f[6][a], byh = W.f[6][addr(b, x, wrap(y + 1, b.n[1]), z)];
-    const float bzl = W.f[7][a], bzh = W.f[7][addr(b, x, y, wrap(z + 1, b.n[2]))];
-    const float b2 = fmaxf(bxl * bxl, bxh * bxh) + fmaxf(byl * byl, byh * byh) +
-                     fmaxf(bzl * bzl, bzh * bzh);
-    const float c = sqrtf((b.gamma * W.f[4][a] + b2) / W.f[0][a]);
-    for (int d = 0; d < 3; ++d) {
-      const float speed = fabsf(W.f[1 + d][a] + c);
-      t = fminf(t, b.cfl * b.d[d] / speed);
-    }
+    t = fminf(t, cell_cfl(b, W, a, x, y, z));
   }
   t = wave_min(t);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
@@ -143,25 +127,11 @@ __global__ void __launch_bounds__(256) k_cfl(Box b, FieldPtrs W, float goal, Ste
   }
 }
 
-__device__ __forceinline__ float cell_cfl(const Box &b, const FieldPtrs &W, long a, int x, int y,
-                                          int z) {
-  const float bxl = W.f[5][a], bxh = W.f[5][addr(b, wrap(x + 1, b.n[0]), y, z)];
-  const float byl = W.f[6][a], byh = W.f[6][addr(b, x, wrap(y + 1, b.n[1]), z)];
-  const float bzl = W.f[7][a], bzh = W.f[7][addr(b, x, y, wrap(z + 1, b.n[2]))];
-  const float b2 = fmaxf(bxl * bxl, bxh * bxh) + fmaxf(byl * byl, byh * byh) +
-                   fmaxf(bzl * bzl, bzh * bzh);
-  const float c = sqrtf((b.gamma * W.f[4][a] + b2) / W.f[0][a]);
-  float t = 3.0e38f;
-  for (int d = 0; d < 3; ++d) t = fminf(t, b.cfl * b.d[d] / fabsf(W.f[1 + d][a] + c));
-  return t;
-}
-
 // CFL over the cells a ghost fill rewrites: the Y ghost rows, and with Z
 // ghosts the Z ghost planes plus the last interior plane (whose upper B_z
 // face is a ghost face).  slab 0: nx * 2My * nz cells, slab 1:
 // nx * ny * (2Mz + 1) cells.
 __global__ void __launch_bounds__(256) k_cfl_ghosts(Box b, FieldPtrs W, int slab, StepState *st) {
-#pragma clang fp contract(off)
   if (st->finished) return;
   __shared__ float red[4];
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -202,32 +172,9 @@ __global__ void __launch_bounds__(256) k_cfl_ghosts(Box b, FieldPtrs W, int slab
   }
 }
 
-// Decide the step: dt = CFL minimum, clipped so that t + dt <= goal.
-__global__ void k_dt(StepState *st, float goal) {
-  if (st->finished) {
-    st->active = 0;
-    return;
-  }
-  float dt = fminf(goal, __uint_as_float(st->cfl_bits));
-  const double next = st->t + (double)dt;
-  if (next > (double)goal) {
-    dt = (float)((double)goal - st->t);
-    st->finished = 1;
-    st->goal_reached = 1;
-  }
-  st->dt = dt;
-  st->h = dt / 2;
-  st->t_next = st->t + (double)dt;
-  st->active = 1;
-}
+__global__ void k_dt(StepState *st, float goal) { step_dt(*st, goal); }
 
-__global__ void k_commit(StepState *st) {
-  if (!st->active) return;
-  st->t = st->t_next;
-  st->generation += 1;
-  st->steps_done += 1;
-  st->active = 0;
-}
+__global__ void k_commit(StepState *st) { step_commit(*st); }
 
 __global__ void k_hlld_batch(long n, float gamma, FluxIn in, FluxOut out) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;

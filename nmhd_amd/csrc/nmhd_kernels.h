@@ -32,6 +32,99 @@ inline unsigned grid_for(long n, int block = kBlock) {
 
 void host_displacement(const Box &b, double t, float *normalized, int *n, float *minor);
 
+// ------------------------------------------------ step record transitions
+// (shared by the single-box and the multi-box bookkeeping kernels)
+
+// Start a step.  full: the CFL minimum is rebuilt over every cell by a full
+// CFL pass; otherwise it starts from the stage-2 epilogue's minimum of the
+// current state (plus, with ghosts, the cells the ghost fill rewrote).
+__device__ __forceinline__ void step_begin(StepState &s, int full) {
+  s.cfl_bits = full ? 0x7f800000u : s.cfl_next_bits;
+  s.cfl_next_bits = 0x7f800000u;  // +inf
+}
+
+// Decide the step: dt = CFL minimum, clipped so that t + dt <= goal
+// (proceed(), INL:1126-1137).
+__device__ __forceinline__ void step_dt(StepState &s, float goal) {
+  if (s.finished) {
+    s.active = 0;
+    return;
+  }
+  float dt = fminf(goal, __uint_as_float(s.cfl_bits));
+  const double next = s.t + (double)dt;
+  if (next > (double)goal) {
+    dt = (float)((double)goal - s.t);
+    s.finished = 1;
+    s.goal_reached = 1;
+  }
+  s.dt = dt;
+  s.h = dt / 2;
+  s.t_next = s.t + (double)dt;
+  s.active = 1;
+}
+
+// GlobalSetup::proceed (mhd_solver.hrb:56-59)
+__device__ __forceinline__ void step_commit(StepState &s) {
+  if (!s.active) return;
+  s.t = s.t_next;
+  s.generation += 1;
+  s.steps_done += 1;
+  s.active = 0;
+}
+
+// CFL time of one cell (cfl_condition_inner, INL:714-720):
+// min_d C*d_d/|v_d + sqrt((gamma p + b2)/rho)|, b2 = sum_d max(B_lo^2, B_hi^2),
+// without FMA contraction so that every CFL pass rounds the same way.
+__device__ __forceinline__ float cell_cfl(const Box &b, const FieldPtrs &W, long a, int x, int y,
+                                          int z) {
+#pragma clang fp contract(off)
+  const float bxl = W.f[5][a], bxh = W.f[5][addr(b, wrap(x + 1, b.n[0]), y, z)];
+  const float byl = W.f[6][a], byh = W.f[6][addr(b, x, wrap(y + 1, b.n[1]), z)];
+  const float bzl = W.f[7][a], bzh = W.f[7][addr(b, x, y, wrap(z + 1, b.n[2]))];
+  const float b2 = fmaxf(bxl * bxl, bxh * bxh) + fmaxf(byl * byl, byh * byh) +
+                   fmaxf(bzl * bzl, bzh * bzh);
+  const float c = sqrtf((b.gamma * W.f[4][a] + b2) / W.f[0][a]);
+  float t = 3.0e38f;
+  for (int d = 0; d < 3; ++d) t = fminf(t, b.cfl * b.d[d] / fabsf(W.f[1 + d][a] + c));
+  return t;
+}
+
+// ------------------------------------------------------- multi-box batches
+
+// One box of a batch: its parameters and the offsets (in floats) of its
+// 8 state fields and 21 face-flux arrays in the batch's pools.
+struct BatchBox {
+  Box b;
+  long off8;
+  long off21;
+};
+
+__host__ __device__ inline FieldPtrs box_fields(float *pool, const BatchBox &bb) {
+  FieldPtrs p;
+  for (int f = 0; f < 8; ++f) p.f[f] = pool + bb.off8 + (long)f * bb.b.ncell;
+  return p;
+}
+
+__host__ __device__ inline FluxSet box_fluxes(float *pool, const BatchBox &bb) {
+  FluxSet F;
+  for (int d = 0; d < 3; ++d)
+    for (int k = 0; k < 7; ++k) F.dir[d].f[k] = pool + bb.off21 + (long)(d * 7 + k) * bb.b.ncell;
+  return F;
+}
+
+// One tile of a persistent launch: box, tile origin, z range, tile shape.
+struct WorkItem {
+  int box;
+  int xy;       // x0 | y0 << 16
+  int z;        // z0 | zend << 16
+  int variant;  // TileVariant
+};
+
+// Tile shapes of the z-marching kernels (TX x TY, 256 threads each).
+enum TileVariant { TV_64x4 = 0, TV_32x8 = 1, TV_16x16 = 2, TV_COUNT = 3 };
+constexpr int kTileTX[TV_COUNT] = {64, 32, 16};
+constexpr int kTileTY[TV_COUNT] = {4, 8, 16};
+
 // kernels_common.hip
 void launch_begin_step(hipStream_t s, StepState *st, bool full);
 void launch_cfl_ghosts(hipStream_t s, const Box &b, FieldPtrs W, StepState *st);
@@ -59,6 +152,33 @@ void launch_flux(hipStream_t s, int dir, const Box &b, FieldPtrs W, FluxOut F,
 void launch_update(hipStream_t s, const Box &b, FieldPtrs Win, const FluxSet &F,
                    FieldPtrs Wbase, FieldPtrs Wout, const StepState *st, int hsel,
                    bool pace_only);
+
+// kernels_batch.hip (persistent multi-box launches over a work table)
+void launch_batch_begin(hipStream_t s, StepState *st, int nbox, bool full);
+void launch_batch_cfl(hipStream_t s, const BatchBox *boxes, int nbox, float *state, float goal,
+                      StepState *st);
+void launch_batch_dt(hipStream_t s, StepState *st, int nbox, const float *goal);
+void launch_batch_commit(hipStream_t s, StepState *st, int nbox);
+void launch_batch_copy(hipStream_t s, const BatchBox *boxes, int nbox, float *src, float *dst,
+                       const StepState *st);
+void launch_batch_faces(hipStream_t s, int grid, const BatchBox *boxes, const WorkItem *items,
+                        int nitems, float *state, float *flux, const StepState *st, int hsel);
+void launch_batch_update(hipStream_t s, int grid, const BatchBox *boxes, const WorkItem *items,
+                         int nitems, float *state_in, float *flux, float *state_base,
+                         float *state_out, StepState *st, int hsel, bool cfl);
+
+// kernels_adjoint.hip (reverse of one RK stage of the periodic step)
+struct AdjStageArgs {
+  FieldPtrs Win, Wbase, Wout;  // stage input, conserved base, stage output
+  FieldPtrs lam_out;           // dL/dW_out
+  FieldPtrs lam_base;          // dL/dW_base (written, or added to with acc_base)
+  FieldPtrs lam_in;            // dL/dW_in through P (written, or added to with acc_in)
+  float h;
+  bool acc_base, acc_in;
+  float *gU[5], *gB[3], *lE[3], *lbb[3], *abn[3], *face[14];  // work arrays (ncell each)
+};
+void launch_adj_set_dt(hipStream_t s, StepState *st, float dt);
+void launch_adj_stage(hipStream_t s, const Box &b, const AdjStageArgs &A);
 
 // kernels_tiled.hip
 int tiled_zchunk(const Box &b);

@@ -187,3 +187,115 @@ def config_c2(n=(256, 256, 256), cfl=0.4):
     """Shearing box n^3 with 4 radial (Y) ghost layers per side."""
     return make_config(n, margin=(0, 4, 0), global_extent=(1.0, 1.0, 1.0),
                        cfl=cfl, source=1, omega=1.0, qshear=1.5, shear_y=1)
+
+
+# ------------------------------------------------------------ box batches
+
+BATCH_EDGES = (16, 24, 32, 48, 64, 96)
+
+
+def batch_shapes(seed=1004, nbox=64, edges=BATCH_EDGES):
+    """Edge lengths (nx, ny, nz) of each box, i.i.d. uniform over `edges`
+    per axis from the counter RNG (component = axis, idx = box)."""
+    out = []
+    for b in range(nbox):
+        e = []
+        for axis in range(3):
+            u = float(uniform01(seed, b, axis))
+            e.append(int(edges[min(int(u * len(edges)), len(edges) - 1)]))
+        out.append(tuple(e))
+    return out
+
+
+def blast(cfg, direction, p_in=10.0, p_out=0.1, radius=0.1, bmag=1.0 / math.sqrt(2.0)):
+    """Blast wave in the periodic unit cube: rho = 1, v = 0, p = p_in within
+    `radius` of the centre (cell centres) and p_out outside, uniform B of
+    magnitude bmag along the unit vector `direction` (face values equal the
+    components, so div B = 0)."""
+    n = cfg["n"]
+    W = empty_state(cfg)
+    x = cell_centres(cfg, 0)
+    y = cell_centres(cfg, 1)
+    z = cell_centres(cfg, 2)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    r2 = (X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2
+    W[0] = 1.0
+    W[4] = np.where(r2 < radius * radius, p_in, p_out)
+    for d in range(3):
+        W[5 + d] = bmag * direction[d]
+    assert W.shape == (NFIELD, n[2], n[1], n[0])
+    return W
+
+
+def batch_boxes(seed=1004, nbox=64, edges=BATCH_EDGES):
+    """The multi-box workload: box b has the edges of batch_shapes; even
+    boxes hold an Orszag-Tang vortex with B0 ~ U[0.5, 1.5]/sqrt(4 pi), odd
+    boxes a blast wave with |B| = 1/sqrt(2) along an isotropic random
+    direction.  Periodic unit cubes, gamma 5/3, CFL 0.4.
+    Returns a list of (cfg, W)."""
+    out = []
+    for b, shape in enumerate(batch_shapes(seed, nbox, edges)):
+        cfg = make_config(shape, cfl=0.4)
+        if b % 2 == 0:
+            b0 = (0.5 + float(uniform01(seed, b, 3))) / math.sqrt(4.0 * math.pi)
+            W = orszag_tang(cfg, b0=b0)
+        else:
+            cos_t = 2.0 * float(uniform01(seed, b, 4)) - 1.0
+            phi = 2.0 * math.pi * float(uniform01(seed, b, 5))
+            sin_t = math.sqrt(max(0.0, 1.0 - cos_t * cos_t))
+            W = blast(cfg, (sin_t * math.cos(phi), sin_t * math.sin(phi), cos_t))
+        out.append((cfg, W))
+    return out
+
+
+# ---------------------------------------------------------- adjoint case
+
+def fourier_modes(seed, count, kmax=4, stream=0):
+    """`count` integer wave vectors k (0 < |k| <= kmax) and phases from the
+    counter RNG (idx = stream*1000 + mode)."""
+    modes = []
+    for m in range(count):
+        j = 0
+        while True:
+            idx = stream * 1000 + m
+            u = [float(uniform01(seed, idx * 64 + j, c)) for c in range(4)]
+            k = tuple(int(math.floor(v * (2 * kmax + 1))) - kmax for v in u[:3])
+            j += 1
+            n2 = k[0] ** 2 + k[1] ** 2 + k[2] ** 2
+            if 0 < n2 <= kmax * kmax:
+                break
+        modes.append((k, 2.0 * math.pi * u[3]))
+    return modes
+
+
+def mode_sum(cfg, modes):
+    """sum_m cos(2 pi k_m . x + phase_m) at the cell centres."""
+    n = cfg["n"]
+    x = cell_centres(cfg, 0)
+    y = cell_centres(cfg, 1)
+    z = cell_centres(cfg, 2)
+    Z, Y, X = np.meshgrid(z, y, x, indexing="ij")
+    out = np.zeros((n[2], n[1], n[0]))
+    for k, ph in modes:
+        out += np.cos(2.0 * math.pi * (k[0] * X + k[1] * Y + k[2] * Z) + ph)
+    return out
+
+
+def adjoint_case(cfg, seed=1005, amp=1e-2, nmodes=8):
+    """Orszag-Tang vortex with a smooth density perturbation: rho times
+    (1 + amp * sum of `nmodes` Fourier modes with |k| <= 4)."""
+    W = orszag_tang(cfg)
+    W[0] = W[0] * (1.0 + amp * mode_sum(cfg, fourier_modes(seed, nmodes)))
+    return W
+
+
+def smooth_direction(cfg, seed, index, fields=(0, 1, 2, 3), nmodes=4):
+    """A smooth random perturbation direction of the initial state: on each
+    listed field a sum of `nmodes` Fourier modes (|k| <= 4) with random
+    signs, scaled to unit RMS; zero on the other fields."""
+    n = cfg["n"]
+    D = np.zeros((NFIELD, n[2], n[1], n[0]))
+    for f in fields:
+        s = mode_sum(cfg, fourier_modes(seed, nmodes, stream=index * 8 + f))
+        D[f] = s / max(np.sqrt((s * s).mean()), 1e-30)
+    return D

@@ -35,11 +35,52 @@ def counters(db):
     return {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in acc.items()}
 
 
+# rocprof kernel names -> per-launch slots of the tiled step
+SLOTS = (("k_faces<", ("faces_s1", "faces_s2")),
+         ("k_update_march<32, 8, false, true>", ("update_s2",)),
+         ("k_update_march<32, 8, false, false>", ("update_s1",)))
+
+
+def traffic(fetch_db, write_db, fetch_scale, write_scale):
+    """HBM bytes per launch per step slot: FETCH_SIZE * fetch_scale +
+    WRITE_SIZE * write_scale (KiB per dispatch -> bytes).  The scales are
+    what the counters read for a copy kernel of known size."""
+    f = counters(fetch_db)
+    w = counters(write_db)
+    out = {}
+    for pat, slots in SLOTS:
+        for k in f:
+            if pat in k and k in w:
+                fk = f[k]["FETCH_SIZE"]
+                wk = w[k]["WRITE_SIZE"]
+                rec = {"rocprof_kernel": k, "fetch_kib": fk, "write_kib": wk,
+                       "fetch_scale": fetch_scale, "write_scale": write_scale,
+                       "bytes_per_launch": 1024.0 * (fk * fetch_scale + wk * write_scale)}
+                for sl in slots:
+                    out[sl] = rec
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("trace")
     ap.add_argument("--pmc", nargs="+", action="append", default=[])
+    ap.add_argument("--traffic-json", help="write HBM bytes per launch (needs --fetch, --write)")
+    ap.add_argument("--fetch")
+    ap.add_argument("--write")
+    ap.add_argument("--fetch-scale", type=float, default=1.0)
+    ap.add_argument("--write-scale", type=float, default=1.0)
+    ap.add_argument("--global-size", type=int, nargs=3, default=(256, 256, 256))
+    ap.add_argument("--path", default="tiled")
+    ap.add_argument("--command", default="")
     a = ap.parse_args()
+    if a.traffic_json:
+        import json
+        rec = {"global_size": list(a.global_size), "path": a.path, "command": a.command,
+               "kernels": traffic(a.fetch, a.write, a.fetch_scale, a.write_scale)}
+        with open(a.traffic_json, "w") as fh:
+            json.dump(rec, fh, indent=1)
+        return
     times = kernel_times(a.trace)
     pmc = {}
     for db in [d for group in a.pmc for d in group]:

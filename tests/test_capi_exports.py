@@ -39,3 +39,15 @@ def test_create_rejects_bad_descriptor_without_gpu():
     rc = capi.load().nmhd_create(ctypes.byref(d), ctypes.byref(h))
     assert rc == 1 and not h.value
     assert b"margin[0]" in capi.load().nmhd_last_error(None)
+
+
+def test_batch_create_rejects_non_periodic_without_gpu():
+    """Batched boxes must be periodic; checked before any device call."""
+    from nmhd_amd import problems as P
+    good = P.make_config((16, 16, 16))
+    bad = P.config_c2((16, 16, 16))
+    import pytest
+    with pytest.raises(capi.NmhdError, match="periodic"):
+        capi.Batch([good, bad])
+    with pytest.raises(capi.NmhdError, match="integrator"):
+        capi.Batch([good, P.make_config((8, 8, 8), integrator=1)])

@@ -1,0 +1,144 @@
+"""GPU checks of the hand-written adjoint (nmhd_adj_* through the torch
+binding nmhd_amd.adjoint):
+
+* the forward step with a frozen dt equals the production step bit for bit;
+* one reverse step equals the FP64 torch-autograd gradient of the plain
+  PyTorch transcription of the step (tests/torch_mhd.py, itself checked
+  against the oracle) within 1e-4 of the gradient's norm per field group;
+* the gradient of the magnetic energy after several steps (through
+  torch.autograd with checkpointing) against the same FP64 reference;
+* every checkpointing scheme gives the same gradient bit for bit;
+* central finite differences (eps = 1e-3) along smooth random directions
+  agree with the adjoint's directional derivatives within 1e-3.
+"""
+import numpy as np
+import pytest
+import torch
+
+import torch_mhd as T
+from nmhd_amd import adjoint as A, capi, problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print("\n[case] " + msg, flush=True)
+
+
+def cuda(W):
+    return torch.tensor(np.ascontiguousarray(W), dtype=torch.float32, device="cuda")
+
+
+def group_norm_err(got, ref):
+    """max over field groups (rho | v | p | B) of |got - ref| / |ref| (2-norms)."""
+    out = []
+    for g in ([0], [1, 2, 3], [4], [5, 6, 7]):
+        r = ref[g].double()
+        out.append(float((got[g].double() - r).norm() / max(float(r.norm()), 1e-30)))
+    return np.array(out)
+
+
+def test_forward_equals_production_step(gpu_ready):
+    announce("adjoint forward == production step (16^3 OT, 3 steps)")
+    cfg = P.config_c1(16)
+    W = P.adjoint_case(cfg)
+    S = A.Stepper(cfg)
+    g = capi.Context(cfg, path=capi.PATH_TILED)
+    g.set_state(W)
+    Wt = cuda(W)
+    for _ in range(3):
+        dt = S.cfl_dt(Wt)
+        Wt = S.forward(Wt, dt)
+        gdt, _ = g.step()
+        assert gdt == dt
+    torch.cuda.synchronize()
+    assert np.array_equal(Wt.cpu().numpy().view(np.uint32), g.get_state().view(np.uint32))
+
+
+@pytest.mark.parametrize("case", ["ot", "blast", "pc_euler"])
+def test_reverse_step_vs_fp64_autograd(gpu_ready, case):
+    announce("one reverse step vs FP64 autograd: " + case)
+    if case == "ot":
+        cfg = P.config_c1(16)
+        W = P.adjoint_case(cfg)
+    elif case == "blast":
+        cfg = P.make_config((16, 12, 8), cfl=0.4)
+        W = P.blast(cfg, (0.48, 0.6, 0.64))
+    else:
+        cfg = P.make_config((12, 16, 8), cfl=0.3, recon=1, integrator=1)
+        W = P.adjoint_case(cfg)
+    S = A.Stepper(cfg)
+    Wt = cuda(W)
+    dt = S.cfl_dt(Wt)
+    rng = np.random.default_rng(1)
+    lam_out = rng.normal(size=W.shape)
+    lam_in = S.reverse(Wt, dt, cuda(lam_out)).cpu()
+    W64 = torch.tensor(W, dtype=torch.float64, requires_grad=True)
+    out = T.step(W64, float(dt), cfg)
+    (out * torch.tensor(lam_out)).sum().backward()
+    errs = group_norm_err(lam_in, W64.grad)
+    assert errs.max() < 1e-4, errs
+
+
+def test_rollout_gradient_vs_fp64_autograd(gpu_ready):
+    announce("8-step magnetic-energy gradient vs FP64 autograd (16^3)")
+    cfg = P.config_c1(16)
+    W = P.adjoint_case(cfg)
+    S = A.Stepper(cfg)
+    W0 = cuda(W).requires_grad_(True)
+    WN, dts = A.rollout(W0, S, nsteps=8, checkpoint="uniform", every=3)
+    (0.5 * (WN[5:8] ** 2).sum()).backward()
+    W64 = torch.tensor(W, dtype=torch.float64, requires_grad=True)
+    X = W64
+    for dt in dts:
+        X = T.step(X, float(dt), cfg)
+    T.magnetic_loss(X).backward()
+    errs = group_norm_err(W0.grad.cpu(), W64.grad)
+    assert errs.max() < 1e-3, errs
+
+
+def test_checkpoint_schemes_bit_identical(gpu_ready):
+    announce("checkpointing all / uniform / revolve give the same gradient")
+    cfg = P.make_config((16, 16, 8), cfl=0.4)
+    W = P.adjoint_case(cfg)
+    S = A.Stepper(cfg)
+    grads = []
+    dts = None
+    for scheme, kw in (("all", {}), ("uniform", {"every": 3}), ("revolve", {"snapshots": 2}),
+                       ("revolve", {"snapshots": 0})):
+        W0 = cuda(W).requires_grad_(True)
+        WN, dts = A.rollout(W0, S, nsteps=10 if dts is None else None, dts=dts,
+                            checkpoint=scheme, **kw)
+        (0.5 * (WN[5:8] ** 2).sum()).backward()
+        grads.append(W0.grad.cpu().numpy())
+    for g in grads[1:]:
+        assert np.array_equal(g.view(np.uint32), grads[0].view(np.uint32))
+
+
+def test_finite_differences_smooth_directions(gpu_ready):
+    announce("central differences (eps 1e-3) on 6 smooth directions, 32^3, 16 steps")
+    cfg = P.config_c1(32)
+    W = P.adjoint_case(cfg)
+    S = A.Stepper(cfg)
+    W0 = cuda(W).requires_grad_(True)
+    WN, dts = A.rollout(W0, S, nsteps=16, checkpoint="uniform", every=4)
+    (0.5 * (WN[5:8].double() ** 2).sum()).backward()
+    grad = W0.grad.double()
+
+    def loss(Wi):
+        X = Wi
+        for dt in dts:
+            X = S.forward(X, dt)
+        return float(0.5 * (X[5:8].double() ** 2).sum())
+
+    eps = 1e-3
+    rel = []
+    for i in range(6):
+        D = P.smooth_direction(cfg, 1006, i)
+        D[0] *= W[0].mean()  # density direction in units of the density
+        Dt = torch.tensor(D, dtype=torch.float64, device="cuda")
+        fd = (loss((cuda(W) + eps * Dt).float().contiguous()) -
+              loss((cuda(W) - eps * Dt).float().contiguous())) / (2 * eps)
+        ad = float((grad * Dt).sum())
+        rel.append(abs(fd - ad) / abs(ad))
+    assert max(rel) < 1e-3, rel

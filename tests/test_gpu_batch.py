@@ -1,0 +1,171 @@
+"""Batched boxes (nmhd_batch_*): many periodic boxes of uneven sizes in
+persistent launches over a work table, per-box dt, HIP-graph replay and the
+autotuner.
+
+The batched kernels run the single-box tile functions, and a cell's
+arithmetic does not depend on the tiling, so the invariant checked here is
+bit-identity: a box advanced in a batch (eager or graph, any tile shape, z
+chunk or workgroup count, tuned or not) equals the same box advanced alone
+by a Context on the tiled path, state and time, bit for bit.  The tiled path
+itself is checked against the oracle in test_gpu_parity.py; one batched box
+is compared with the oracle here as well.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from mhd_checks import bitwise_equal, div_b, group_rel_err
+from nmhd_amd import capi, problems as P
+
+pytestmark = pytest.mark.gpu
+
+SMALL = [(16, 24, 8), (32, 16, 16), (24, 24, 24), (48, 16, 8), (16, 16, 32), (64, 8, 12)]
+
+
+def announce(msg):
+    print("\n[case] " + msg, flush=True)
+
+
+def small_boxes(seed=11):
+    out = []
+    for b, shape in enumerate(SMALL):
+        cfg = P.make_config(shape, cfl=0.4)
+        if b % 2 == 0:
+            W = P.orszag_tang_noisy(cfg, seed=seed + b, b0=0.3 + 0.05 * b)
+        else:
+            u = P.uniform01(seed, b, 0)
+            W = P.blast(cfg, (float(np.cos(6.0 * u)), float(np.sin(6.0 * u)), 0.3))
+        out.append((cfg, W))
+    return out
+
+
+def make_batch(boxes):
+    B = capi.Batch([c for c, _ in boxes])
+    for i, (_, W) in enumerate(boxes):
+        B.set_state(i, W)
+    return B
+
+
+def single_runs(boxes, goal, nsteps):
+    out = []
+    for cfg, W in boxes:
+        g = capi.Context(cfg, path=capi.PATH_TILED)
+        g.set_state(W)
+        k, _ = g.run(goal, nsteps)
+        out.append((g.get_state(), g.get_time(), k))
+        g.close()
+    return out
+
+
+def assert_batch_equals(B, ref):
+    for i, (W, (t, gen), k) in enumerate(ref):
+        got = B.get_state(i)
+        assert bitwise_equal(got, W), "box %d state differs" % i
+        tb, gb = B.get_time(i)
+        assert tb == t and gb == gen, (i, tb, t, gb, gen)
+
+
+def test_batch_eager_equals_single_boxes(gpu_ready):
+    announce("batch eager 6 boxes x 5 steps vs single contexts")
+    boxes = small_boxes()
+    ref = single_runs(boxes, 1e30, 5)
+    B = make_batch(boxes)
+    k, reached = B.run(1e30, 5, use_graph=False)
+    assert k == 5 and not reached
+    assert_batch_equals(B, ref)
+    # the boxes took different dt
+    dts = {B.status(i)[2] for i in range(B.nbox)}
+    assert len(dts) > 1
+
+
+def test_batch_graph_equals_eager(gpu_ready):
+    announce("batch graph vs eager, 7 + 4 steps (every rotation phase, replays)")
+    boxes = small_boxes(seed=3)
+    A = make_batch(boxes)
+    G = make_batch(boxes)
+    A.run(1e30, 7, use_graph=False)
+    G.run(1e30, 7, use_graph=True)
+    for i in range(A.nbox):
+        assert bitwise_equal(A.get_state(i), G.get_state(i)), i
+    A.run(1e30, 4, use_graph=False)
+    G.run(1e30, 4, use_graph=True)
+    ref = single_runs(boxes, 1e30, 11)
+    assert_batch_equals(G, ref)
+    assert_batch_equals(A, ref)
+
+
+@pytest.mark.parametrize("ft,ut,fz,uz,fw,uw", [
+    (capi.TILE_64x4, capi.TILE_64x4, 8, 32, 1, 4),
+    (capi.TILE_32x8, capi.TILE_16x16, 5, 7, 3, 1),
+    (capi.TILE_16x16, capi.TILE_32x8, 32, 3, 2, 2),
+])
+def test_batch_tilings_bit_identical(gpu_ready, ft, ut, fz, uz, fw, uw):
+    announce("batch tiling faces %d update %d zc %d/%d wg %d/%d" % (ft, ut, fz, uz, fw, uw))
+    boxes = small_boxes(seed=5)
+    ref = single_runs(boxes, 1e30, 3)
+    B = make_batch(boxes)
+    B.set_tiles(ft, ut)
+    B.set_launch(fz, uz, fw, uw)
+    B.run(1e30, 3, use_graph=True)
+    assert_batch_equals(B, ref)
+
+
+def test_batch_per_box_goal_clipping(gpu_ready):
+    announce("batch goal clipping: boxes reach the goal at different steps")
+    boxes = small_boxes(seed=8)
+    # a goal that some boxes reach within 6 steps and others do not
+    probe = make_batch(boxes)
+    probe.run(1e30, 1, use_graph=False)
+    dts = sorted(probe.status(i)[2] for i in range(probe.nbox))
+    goal = float(np.float32(3.5 * dts[len(dts) // 2]))
+    ref = single_runs(boxes, goal, 6)
+    B = make_batch(boxes)
+    kmin, all_reached = B.run(goal, 6, use_graph=True)
+    assert_batch_equals(B, ref)
+    reached = [B.status(i)[1] for i in range(B.nbox)]
+    assert any(reached) and not all(reached)
+    for i, r in enumerate(reached):
+        t, _ = B.get_time(i)
+        if r:
+            assert t == pytest.approx(goal, rel=0, abs=1e-6)
+    assert kmin == min(k for _, _, k in ref)
+
+
+def test_batch_autotune_cached_and_identical(gpu_ready, tmp_path):
+    announce("batch autotune (cache in tmp) and results after tuning")
+    boxes = small_boxes(seed=9)
+    ref = single_runs(boxes, 1e30, 3)
+    B = make_batch(boxes)
+    cache = str(tmp_path / "tune.txt")
+    n1 = B.autotune(cache)
+    assert n1 == len(set(SMALL))
+    assert os.path.exists(cache)
+    text = open(cache).read()
+    assert text.count("\nshape ") == len(set(SMALL)) and "\nlaunch " in text
+    B2 = make_batch(boxes)
+    assert B2.autotune(cache) == 0  # every shape from the cache
+    assert B2.get_launch() == B.get_launch()
+    B.run(1e30, 3, use_graph=True)
+    assert_batch_equals(B, ref)
+
+
+def test_batch_box_against_oracle(gpu_ready):
+    announce("batched box vs oracle, 3 steps")
+    cfg = P.make_config((16, 24, 8), cfl=0.4)
+    W = P.orszag_tang_noisy(cfg, seed=21)
+    other_cfg = P.make_config((32, 16, 16), cfl=0.4)
+    B = capi.Batch([other_cfg, cfg])
+    B.set_state(0, P.orszag_tang(other_cfg))
+    B.set_state(1, W)
+    B.run(1e30, 3, use_graph=True)
+    o = po.Oracle(cfg, W)
+    o.run(1e30, 3)
+    got, want = B.get_state(1), o.get_state()
+    errs = group_rel_err(got, want)
+    assert errs.max() < 1e-3, errs
+    assert B.get_time(1)[0] == pytest.approx(o.time, rel=1e-6)
+    bmax = np.abs(got[5:8]).max()
+    assert np.abs(div_b(got, cfg["d"])).max() * max(cfg["d"]) < 1e-6 * bmax
+
