@@ -73,6 +73,13 @@ enum nmhd_path {
   NMHD_PATH_STRICT = 3
 };
 
+/* Tile shapes (TX x TY columns marched along z) of the tiled kernels. */
+enum nmhd_tile {
+  NMHD_TILE_64x4 = 0,
+  NMHD_TILE_32x8 = 1,
+  NMHD_TILE_16x16 = 2
+};
+
 /* Box description; the compile-time constants of the reference's
  * simulation_config.h (simulation_config_default.h:25-122) as run-time
  * values.  Derived constants (kExtent*, kD*, kLocalExtent*) are computed in
@@ -187,6 +194,11 @@ int nmhd_timing_get(nmhd_ctx *ctx, int i, const char **name, double *total_ms,
 /* Select the kernel path (enum nmhd_path) for later steps. */
 int nmhd_set_path(nmhd_ctx *ctx, int path);
 int nmhd_get_path(const nmhd_ctx *ctx, int *path);
+/* Tile shapes (enum nmhd_tile) of the face and update kernels of
+ * NMHD_PATH_TILED (default NMHD_TILE_16x16 and NMHD_TILE_64x4); results do
+ * not depend on them. */
+int nmhd_set_tiles(nmhd_ctx *ctx, int faces_tile, int update_tile);
+int nmhd_get_tiles(const nmhd_ctx *ctx, int *faces_tile, int *update_tile);
 
 /* ------------------------------------------------------------------------
  * Batches: many independent periodic boxes of uneven sizes advanced
@@ -201,13 +213,6 @@ int nmhd_get_path(const nmhd_ctx *ctx, int *path);
  * and all boxes share one integrator.
  * ---------------------------------------------------------------------- */
 typedef struct nmhd_batch nmhd_batch;
-
-/* Tile shapes (TX x TY columns marched along z) of the batched kernels. */
-enum nmhd_tile {
-  NMHD_TILE_64x4 = 0,
-  NMHD_TILE_32x8 = 1,
-  NMHD_TILE_16x16 = 2
-};
 
 int nmhd_batch_create(const nmhd_desc *descs, int nbox, nmhd_batch **out);
 int nmhd_batch_destroy(nmhd_batch *batch);
@@ -262,8 +267,8 @@ int nmhd_batch_run(nmhd_batch *batch, float goal_time, int nsteps, int use_graph
  * proceed()'s loop (INL:1138-1199) for margin {0,0,0}, source NONE.  State
  * and adjoint arrays are caller-owned DEVICE arrays (8 fields, ncell floats
  * each, same layout as nmhd_set_state); work is enqueued on the stream set
- * by nmhd_adj_set_stream (default: the object's own) and is asynchronous
- * unless stated.  The reverse of the minmod limiter, of the upwind CT choice
+ * by nmhd_adj_set_stream and is asynchronous unless stated, so the caller's
+ * own work on that stream is ordered with it.  The reverse of the minmod limiter, of the upwind CT choice
  * and of the HLLD fan follow the branches the forward takes at w_in.
  * ---------------------------------------------------------------------- */
 typedef struct nmhd_adj nmhd_adj;
@@ -272,8 +277,10 @@ int nmhd_adj_create(const nmhd_desc *desc, nmhd_adj **out);
 int nmhd_adj_destroy(nmhd_adj *adj);
 const char *nmhd_adj_last_error(const nmhd_adj *adj);
 int nmhd_adj_get_geometry(const nmhd_adj *adj, nmhd_geometry *geo);
-/* stream: a hipStream_t (e.g. torch.cuda.current_stream().cuda_stream), or
- * NULL for the object's own stream. */
+/* Launch later work on `stream`, a hipStream_t such as
+ * torch.cuda.current_stream().cuda_stream; NULL selects HIP's null stream
+ * (torch's default stream).  Until the first call the object uses a stream
+ * of its own. */
 int nmhd_adj_set_stream(nmhd_adj *adj, void *stream);
 int nmhd_adj_synchronize(nmhd_adj *adj);
 /* The CFL dt of a device state (as nmhd_cfl_dt).  Synchronises. */

@@ -22,7 +22,10 @@ SOURCES = ["capi.hip", "capi_batch.hip", "capi_adjoint.hip", "kernels_common.hip
 FILE_FLAGS = {"kernels_strict.hip": ["-ffp-contract=off"]}
 
 HIPFLAGS = [
-    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+    # -ffp-contract=on: FMA contraction only within one source expression, so
+    # the same expression rounds the same way in every kernel that inlines it
+    # (a cell's result does not depend on the tiling or the kernel variant)
+    "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=on",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics", "-Wall", "-Wno-unused-function",
     "-I", os.path.join(ROOT, "include"), "-I", CSRC,

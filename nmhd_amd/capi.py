@@ -24,7 +24,7 @@ EXPORTS = (
     "nmhd_step", "nmhd_run", "nmhd_reset_run", "nmhd_enqueue_steps",
     "nmhd_synchronize", "nmhd_hlld_batch", "nmhd_timing_enable",
     "nmhd_timing_reset", "nmhd_timing_get", "nmhd_set_path", "nmhd_get_path",
-    "nmhd_batch_create", "nmhd_batch_destroy", "nmhd_batch_last_error",
+    "nmhd_set_tiles", "nmhd_get_tiles", "nmhd_batch_create", "nmhd_batch_destroy", "nmhd_batch_last_error",
     "nmhd_batch_get_geometry", "nmhd_batch_set_state", "nmhd_batch_get_state",
     "nmhd_batch_set_time", "nmhd_batch_get_time", "nmhd_batch_get_status",
     "nmhd_batch_set_tiles", "nmhd_batch_set_launch", "nmhd_batch_get_launch",
@@ -108,6 +108,8 @@ def load():
                             ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_long)],
         "nmhd_set_path": [_VP, ctypes.c_int],
         "nmhd_get_path": [_VP, _IP],
+        "nmhd_set_tiles": [_VP, ctypes.c_int, ctypes.c_int],
+        "nmhd_get_tiles": [_VP, _IP, _IP],
         "nmhd_batch_create": [ctypes.POINTER(NmhdDesc), ctypes.c_int, ctypes.POINTER(_VP)],
         "nmhd_batch_destroy": [_VP],
         "nmhd_batch_get_geometry": [_VP, ctypes.c_int, ctypes.POINTER(NmhdGeometry)],
@@ -277,6 +279,15 @@ class Context:
         p = ctypes.c_int()
         self._c(load().nmhd_get_path(self.h, ctypes.byref(p)))
         return p.value
+
+    def set_tiles(self, faces_tile, update_tile):
+        self._c(load().nmhd_set_tiles(self.h, faces_tile, update_tile))
+
+    def get_tiles(self):
+        f = ctypes.c_int()
+        u = ctypes.c_int()
+        self._c(load().nmhd_get_tiles(self.h, ctypes.byref(f), ctypes.byref(u)))
+        return f.value, u.value
 
     def timing_enable(self, on=True):
         self._c(load().nmhd_timing_enable(self.h, int(on)))

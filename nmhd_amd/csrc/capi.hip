@@ -125,6 +125,8 @@ struct nmhd_ctx {
   StepState *st = nullptr;
   StepState *diag = nullptr;
   int path = NMHD_PATH_TILED;
+  // tile shapes of the tiled path (fastest measured at 256^3 on MI355X)
+  int ftile = TV_16x16, utile = TV_64x4;
   bool cfl_valid = false;  // st->cfl_next_bits holds the CFL minimum of the state
   std::string err;
   bool timing = false;
@@ -240,10 +242,10 @@ void enqueue_step(nmhd_ctx *c, float goal) {
   auto stage = [&](int in, int base, int out, int hsel, bool last) {
     if (tiled) {
       timed(c, last ? S_FACES2 : S_FACES1,
-            [&] { launch_faces(s, b, fields(c, in), F, c->st, hsel); });
+            [&] { launch_faces(s, b, fields(c, in), F, c->st, hsel, c->ftile); });
       timed(c, last ? S_UPDATE2 : S_UPDATE1, [&] {
         launch_update_tiled(s, b, fields(c, in), F, fields(c, base), fields(c, out), c->st, hsel,
-                            false, last);
+                            false, last, c->utile);
       });
       return;
     }
@@ -470,9 +472,9 @@ int nmhd_rhs(nmhd_ctx *c, float h, float *const pace[8]) {
   launch_diag_prepare(c->stream, c->diag, c->st, h);
   const FluxSet F = fluxes(c);
   if (c->path == NMHD_PATH_TILED) {
-    launch_faces(c->stream, c->box, fields(c, c->cur), F, c->diag, H_DIAG);
+    launch_faces(c->stream, c->box, fields(c, c->cur), F, c->diag, H_DIAG, c->ftile);
     launch_update_tiled(c->stream, c->box, fields(c, c->cur), F, fields(c, c->cur), out, c->diag,
-                        H_DIAG, true, false);
+                        H_DIAG, true, false, c->utile);
   } else {
     for (int d = 0; d < 3; ++d)
       flux_dispatch(c, d, fields(c, c->cur), F.dir[d], c->diag, H_DIAG);
@@ -602,6 +604,23 @@ int nmhd_set_path(nmhd_ctx *c, int path) {
   if (!c || path < NMHD_PATH_AUTO || path > NMHD_PATH_STRICT) return NMHD_EINVAL;
   c->path = resolve_path(path);
   c->cfl_valid = false;
+  return NMHD_OK;
+}
+
+int nmhd_set_tiles(nmhd_ctx *c, int faces_tile, int update_tile) {
+  if (!c || faces_tile < 0 || faces_tile >= TV_COUNT || update_tile < 0 ||
+      update_tile >= TV_COUNT)
+    return NMHD_EINVAL;
+  int r = sync_and_check(c);
+  c->ftile = faces_tile;
+  c->utile = update_tile;
+  return r;
+}
+
+int nmhd_get_tiles(const nmhd_ctx *c, int *faces_tile, int *update_tile) {
+  if (!c) return NMHD_EINVAL;
+  if (faces_tile) *faces_tile = c->ftile;
+  if (update_tile) *update_tile = c->utile;
   return NMHD_OK;
 }
 

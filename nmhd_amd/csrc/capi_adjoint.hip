@@ -181,8 +181,8 @@ int nmhd_adj_get_geometry(const nmhd_adj *a, nmhd_geometry *geo) {
 
 int nmhd_adj_set_stream(nmhd_adj *a, void *stream) {
   if (!a) return NMHD_EINVAL;
-  AHIP(a, hipStreamSynchronize(a->stream));
-  a->stream = stream ? (hipStream_t)stream : a->own;
+  if (a->stream != (hipStream_t)stream) AHIP(a, hipStreamSynchronize(a->stream));
+  a->stream = (hipStream_t)stream;  // NULL: HIP's null stream
   return NMHD_OK;
 }
 

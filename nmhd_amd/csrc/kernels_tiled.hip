@@ -15,7 +15,7 @@ namespace nmhd {
 namespace tiled {
 
 template <int TX, int TY>
-__global__ void __launch_bounds__(kTileThreads) k_faces(Box b, FieldPtrs W, FluxSet F,
+__global__ void __launch_bounds__(kTileThreads, 4) k_faces(Box b, FieldPtrs W, FluxSet F,
                                                         const StepState *st, int hsel, int zc,
                                                         int gx, int gy) {
   if (!st->active) return;
@@ -54,19 +54,20 @@ int tiled_zchunk(const Box &b) {
   return zc;
 }
 
-void launch_faces(hipStream_t s, const Box &b, FieldPtrs W, const FluxSet &F, const StepState *st,
-                  int hsel) {
+namespace {
+
+template <int TX, int TY>
+void faces_as(hipStream_t s, const Box &b, FieldPtrs W, const FluxSet &F, const StepState *st,
+              int hsel) {
   const int zc = tiled_zchunk(b);
-  constexpr int TX = 64, TY = 4;
   const int gx = (b.n[0] + TX - 1) / TX, gy = (b.n[1] + TY - 1) / TY, gz = (b.n[2] + zc - 1) / zc;
   hipLaunchKernelGGL((tiled::k_faces<TX, TY>), dim3(gx * gy * gz), dim3(tiled::kTileThreads), 0,
                      s, b, W, F, st, hsel, zc, gx, gy);
 }
 
-void launch_update_tiled(hipStream_t s, const Box &b, FieldPtrs Win, const FluxSet &F,
-                         FieldPtrs Wbase, FieldPtrs Wout, StepState *st, int hsel, bool pace_only,
-                         bool cfl) {
-  constexpr int TX = 32, TY = 8;
+template <int TX, int TY>
+void update_as(hipStream_t s, const Box &b, FieldPtrs Win, const FluxSet &F, FieldPtrs Wbase,
+               FieldPtrs Wout, StepState *st, int hsel, bool pace_only, bool cfl) {
   const int zc = tiled_zchunk(b);
   const int gx = (b.n[0] + TX - 1) / TX, gy = (b.n[1] + TY - 1) / TY, gz = (b.n[2] + zc - 1) / zc;
   const dim3 grid(gx * gy * gz), blk(tiled::kTileThreads);
@@ -79,6 +80,29 @@ void launch_update_tiled(hipStream_t s, const Box &b, FieldPtrs Win, const FluxS
   else
     hipLaunchKernelGGL((tiled::k_update_march<TX, TY, false, false>), grid, blk, 0, s, b, Win, F,
                        Wbase, Wout, st, hsel, zc, gx, gy);
+}
+
+}  // namespace
+
+void launch_faces(hipStream_t s, const Box &b, FieldPtrs W, const FluxSet &F, const StepState *st,
+                  int hsel, int tile) {
+  if (tile == TV_64x4)
+    faces_as<64, 4>(s, b, W, F, st, hsel);
+  else if (tile == TV_16x16)
+    faces_as<16, 16>(s, b, W, F, st, hsel);
+  else
+    faces_as<32, 8>(s, b, W, F, st, hsel);
+}
+
+void launch_update_tiled(hipStream_t s, const Box &b, FieldPtrs Win, const FluxSet &F,
+                         FieldPtrs Wbase, FieldPtrs Wout, StepState *st, int hsel, bool pace_only,
+                         bool cfl, int tile) {
+  if (tile == TV_64x4)
+    update_as<64, 4>(s, b, Win, F, Wbase, Wout, st, hsel, pace_only, cfl);
+  else if (tile == TV_16x16)
+    update_as<16, 16>(s, b, Win, F, Wbase, Wout, st, hsel, pace_only, cfl);
+  else
+    update_as<32, 8>(s, b, Win, F, Wbase, Wout, st, hsel, pace_only, cfl);
 }
 
 }  // namespace nmhd

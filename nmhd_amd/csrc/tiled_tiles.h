@@ -3,14 +3,17 @@
 // persistent multi-box launches (kernels_batch.hip).
 //
 //  faces_tile   all three face fluxes of every cell of a TX x TY column tile
-//               over planes [z0, zend), one HLLD per face and no halo
-//               recomputation.  The x and y stencils (cells i-2..i+1) are
-//               read from an LDS ring of two planes that the workgroup stages
-//               cooperatively; the z stencil (planes k-2..k+1) lives in a
-//               per-thread register queue filled as the march passes each
-//               plane.  Writes the seven face fluxes of each direction.
-//  update_tile  per cell of the tile: flux divergence, the 24 upwinded edge
-//               EMFs of the cell's six faces, the conserved update, primitive
+//               over planes [z0, zend), one HLLD per face plus the tile's
+//               upper boundary faces.  The x and y stencils (cells i-2..i+1)
+//               are read from an LDS ring of two planes that the workgroup
+//               stages cooperatively; the z stencil (planes k-2..k+1) lives
+//               in a per-thread register queue filled as the march passes
+//               each plane.  Differences the hydro fluxes in the tile (LDS
+//               for x and y, registers for z) and writes the hydro
+//               divergence of every cell (5 floats) and the six B fluxes the
+//               CT needs (6 floats).
+//  update_tile  per cell of the tile: the 24 upwinded edge EMFs of the
+//               cell's six faces, the conserved update, primitive
 //               recovery and the shearing-box kick, as a z march with the
 //               neighbour data staged in LDS rings one plane ahead; in stage
 //               2 it also returns the CFL time of the new state of the
@@ -81,20 +84,58 @@ __device__ __forceinline__ void block_min_into(float v, unsigned *target, float 
 
 constexpr int kTileThreads = 256;
 
+// What the face kernel hands to the update kernel, in the 21-array flux
+// storage of a stage: the hydro divergence of every cell (5 arrays, in the
+// slots of the x-sweep's hydro fluxes) and the six face fluxes of B that
+// the CT edge EMFs need.
+struct StageBuf {
+  float *pace[5];  // P_rho, P_mx, P_my, P_mz, P_E
+  float *fb[6];    // x-face F(B_y), x-face F(B_z), y-face F(B_z), y-face F(B_x),
+                   // z-face F(B_x), z-face F(B_y)
+};
+
+__host__ __device__ inline StageBuf stage_buf(const FluxSet &F) {
+  StageBuf s;
+  for (int k = 0; k < 5; ++k) s.pace[k] = F.dir[0].f[k];
+  s.fb[0] = F.dir[0].f[5];
+  s.fb[1] = F.dir[0].f[6];
+  s.fb[2] = F.dir[1].f[5];
+  s.fb[3] = F.dir[1].f[6];
+  s.fb[4] = F.dir[2].f[5];
+  s.fb[5] = F.dir[2].f[6];
+  return s;
+}
+
 template <int TX, int TY>
 struct FacesShape {
   static_assert(TX * TY == kTileThreads, "tiles have 256 threads");
   static constexpr int LX = TX + 4, LY = TY + 4, LN = LX * LY;
   static constexpr int NS = (LN + kTileThreads - 1) / kTileThreads;  // slots per thread
-  static constexpr int kLds = 2 * 8 * LN;                              // floats
+  // LDS (floats): two staged planes of the 8 fields, then the hydro fluxes
+  // of the plane's x-faces [5][TY][TX+1] and y-faces [5][TY+1][TX]
+  static constexpr int kPlanes = 2 * 8 * LN;
+  static constexpr int kFx = kPlanes;
+  static constexpr int kFy = kFx + 5 * TY * (TX + 1);
+  static constexpr int kLds = kFy + 5 * (TY + 1) * TX;
 };
 
+// Face fluxes of a TX x TY column tile over planes [z0, zend), and the
+// hydro divergence of its cells.  Per plane p the workgroup computes the
+// x-faces i in [x0, x0+TX] and the y-faces j in [y0, y0+TY] (the upper
+// boundary faces of the tile, i.e. one face column and one face row more
+// than it owns, spread over the first TY+TX threads), keeps their hydro
+// fluxes in LDS and differences them; the z-faces come from the register
+// queue, one plane of faces beyond zend.  Only the tile's own faces write
+// their B fluxes to memory.
 template <int TX, int TY>
-__device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, const FluxSet &F,
+__device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, const FluxSet &Fs,
                                            float h, int x0, int y0, int z0, int zend,
                                            float *lds) {
   using S = FacesShape<TX, TY>;
   constexpr int NT = kTileThreads, LX = S::LX, LN = S::LN, NS = S::NS;
+  const StageBuf SB = stage_buf(Fs);
+  float *const FXL = lds + S::kFx;
+  float *const FYL = lds + S::kFy;
   const int tid = threadIdx.x, tx = tid % TX, ty = tid / TX;
   const int nx = b.n[0], ny = b.n[1], nz = b.n[2];
   const int plane = nx * ny;
@@ -102,6 +143,7 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
   const bool valid = (x < nx) && (y < ny);
   const int mode = b.recon;
   const bool src = b.source == 1;
+  const float idx = b.inv_d[0], idy = b.inv_d[1], idz = b.inv_d[2];
 
   // global in-plane offset of every LDS slot this thread stages
   int gxy[NS];
@@ -133,126 +175,194 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
     }
   };
 
+  // lower x-face of the cell at LDS index c of plane buffer cb (nb: next plane)
+  auto face_x = [&](int cb, int nb, int c) -> Flux7 {
+    const float *R = lds + (cb * 8 + 0) * LN, *VX = lds + (cb * 8 + 1) * LN;
+    const float *VY = lds + (cb * 8 + 2) * LN, *VZ = lds + (cb * 8 + 3) * LN;
+    const float *PR = lds + (cb * 8 + 4) * LN, *BX = lds + (cb * 8 + 5) * LN;
+    const float *BY = lds + (cb * 8 + 6) * LN, *BZ = lds + (cb * 8 + 7) * LN;
+    const float *BZN = lds + (nb * 8 + 7) * LN;
+    float L[7], Rr[7];
+    const int s0 = c - 2, s1 = c - 1, s2 = c, s3 = c + 1;
+    recon(mode, R[s0], R[s1], R[s2], R[s3], L[0], Rr[0]);
+    recon(mode, VX[s0], VX[s1], VX[s2], VX[s3], L[1], Rr[1]);
+    recon(mode, VY[s0], VY[s1], VY[s2], VY[s3], L[2], Rr[2]);
+    recon(mode, VZ[s0], VZ[s1], VZ[s2], VZ[s3], L[3], Rr[3]);
+    recon(mode, PR[s0], PR[s1], PR[s2], PR[s3], L[4], Rr[4]);
+    recon(mode, 0.5f * (BY[s0] + BY[s0 + LX]), 0.5f * (BY[s1] + BY[s1 + LX]),
+          0.5f * (BY[s2] + BY[s2 + LX]), 0.5f * (BY[s3] + BY[s3 + LX]), L[5], Rr[5]);
+    recon(mode, 0.5f * (BZ[s0] + BZN[s0]), 0.5f * (BZ[s1] + BZN[s1]), 0.5f * (BZ[s2] + BZN[s2]),
+          0.5f * (BZ[s3] + BZN[s3]), L[6], Rr[6]);
+    if (src) {
+      const float mbv = 0.5f * h * accel_x(b, 0.5f * (VY[s1] + VY[s2]));
+      L[1] += mbv;
+      Rr[1] += mbv;
+    }
+    return hlld(b.gamma, b.inv_gm1, BX[c], L[0], Rr[0], L[1], Rr[1], L[2], Rr[2], L[3], Rr[3],
+                L[4], Rr[4], L[5], Rr[5], L[6], Rr[6]);
+  };
+  // lower y-face of the cell at LDS index c; yrow = the face's array row
+  auto face_y = [&](int cb, int nb, int c, int yrow) -> Flux7 {
+    const float *R = lds + (cb * 8 + 0) * LN, *VX = lds + (cb * 8 + 1) * LN;
+    const float *VY = lds + (cb * 8 + 2) * LN, *VZ = lds + (cb * 8 + 3) * LN;
+    const float *PR = lds + (cb * 8 + 4) * LN, *BX = lds + (cb * 8 + 5) * LN;
+    const float *BY = lds + (cb * 8 + 6) * LN, *BZ = lds + (cb * 8 + 7) * LN;
+    const float *BZN = lds + (nb * 8 + 7) * LN;
+    float L[7], Rr[7];
+    const int s0 = c - 2 * LX, s1 = c - LX, s2 = c, s3 = c + LX;
+    recon(mode, R[s0], R[s1], R[s2], R[s3], L[0], Rr[0]);
+    recon(mode, VY[s0], VY[s1], VY[s2], VY[s3], L[1], Rr[1]);
+    recon(mode, VZ[s0], VZ[s1], VZ[s2], VZ[s3], L[2], Rr[2]);
+    recon(mode, VX[s0], VX[s1], VX[s2], VX[s3], L[3], Rr[3]);
+    recon(mode, PR[s0], PR[s1], PR[s2], PR[s3], L[4], Rr[4]);
+    recon(mode, 0.5f * (BZ[s0] + BZN[s0]), 0.5f * (BZ[s1] + BZN[s1]), 0.5f * (BZ[s2] + BZN[s2]),
+          0.5f * (BZ[s3] + BZN[s3]), L[5], Rr[5]);
+    recon(mode, 0.5f * (BX[s0] + BX[s0 + 1]), 0.5f * (BX[s1] + BX[s1 + 1]),
+          0.5f * (BX[s2] + BX[s2 + 1]), 0.5f * (BX[s3] + BX[s3 + 1]), L[6], Rr[6]);
+    if (src) {
+      const float mbv = 0.5f * h * accel_y(b, 0.5f * (VX[s1] + VX[s2]), yrow);
+      L[1] += mbv;
+      Rr[1] += mbv;
+    }
+    return hlld(b.gamma, b.inv_gm1, BY[c], L[0], Rr[0], L[1], Rr[1], L[2], Rr[2], L[3], Rr[3],
+                L[4], Rr[4], L[5], Rr[5], L[6], Rr[6]);
+  };
+  // hydro fluxes in the conserved order (rho, m_x, m_y, m_z, E)
+  auto put_x = [&](int row, int col, const Flux7 &r) {
+    float *d = FXL + row * (TX + 1) + col;
+    constexpr int st = TY * (TX + 1);
+    d[0] = r.rho;
+    d[st] = r.mn;
+    d[2 * st] = r.mt1;
+    d[3 * st] = r.mt2;
+    d[4 * st] = r.e;
+  };
+  auto put_y = [&](int row, int col, const Flux7 &r) {
+    float *d = FYL + row * TX + col;
+    constexpr int st = (TY + 1) * TX;
+    d[0] = r.rho;
+    d[st] = r.mt2;
+    d[2 * st] = r.mn;
+    d[3 * st] = r.mt1;
+    d[4 * st] = r.e;
+  };
+
   // own cell in the LDS tile
   const int c = (ty + 2) * LX + tx + 2;
   // z-stencil queue: q[s] holds plane p-3+s: rho, vz, vx, vy, p, Bx_avg, By_avg
   float q[4][7];
   float bz_prev = 0.0f;  // B_z on the lower z-face of plane p-1
   float bz_cur = 0.0f;
+  // divergence pipeline: x+y part of planes p-1, p-2; z-flux of the lower
+  // face of plane p-2
+  float pxy1[5] = {}, pxy2[5] = {}, fzl[5] = {};
 
-  const int zs = z0 - 2;
+  const int zs = z0 - 2, zlast = zend + 1;
   load_plane(zs);
   store_plane(zs & 1);
   load_plane(zs + 1);
   store_plane((zs + 1) & 1);
   __syncthreads();
 
-  for (int p = zs; p <= zend; ++p) {
+  for (int p = zs; p <= zlast; ++p) {
     const int cb = p & 1, nb = (p + 1) & 1;
-    const bool prefetch = (p + 2 <= zend);
+    const bool prefetch = (p + 2 <= zlast);
     if (prefetch) load_plane(p + 2);
-    const float *R = lds + (cb * 8 + 0) * LN;
-    const float *VX = lds + (cb * 8 + 1) * LN;
-    const float *VY = lds + (cb * 8 + 2) * LN;
-    const float *VZ = lds + (cb * 8 + 3) * LN;
-    const float *PR = lds + (cb * 8 + 4) * LN;
-    const float *BX = lds + (cb * 8 + 5) * LN;
-    const float *BY = lds + (cb * 8 + 6) * LN;
-    const float *BZ = lds + (cb * 8 + 7) * LN;
-    const float *BZN = lds + (nb * 8 + 7) * LN;
-
-    // push plane p into the z queue
+    {
+      const float *P0 = lds + cb * 8 * LN;
+      // push plane p into the z queue
 #pragma unroll
-    for (int s = 0; s < 3; ++s)
+      for (int s = 0; s < 3; ++s)
 #pragma unroll
-      for (int v = 0; v < 7; ++v) q[s][v] = q[s + 1][v];
-    q[3][0] = R[c];
-    q[3][1] = VZ[c];
-    q[3][2] = VX[c];
-    q[3][3] = VY[c];
-    q[3][4] = PR[c];
-    q[3][5] = 0.5f * (BX[c] + BX[c + 1]);
-    q[3][6] = 0.5f * (BY[c] + BY[c + LX]);
-    bz_prev = bz_cur;
-    bz_cur = BZ[c];
-
-    if (valid && p >= z0 && p < zend) {
-      const int a = wrap(p, nz) * plane + y * nx + x;
-      // ---- x-face (lower x-face of cell (x, y, p))
-      {
-        float L[7], Rr[7];
-        const int s0 = c - 2, s1 = c - 1, s2 = c, s3 = c + 1;
-        recon(mode, R[s0], R[s1], R[s2], R[s3], L[0], Rr[0]);
-        recon(mode, VX[s0], VX[s1], VX[s2], VX[s3], L[1], Rr[1]);
-        recon(mode, VY[s0], VY[s1], VY[s2], VY[s3], L[2], Rr[2]);
-        recon(mode, VZ[s0], VZ[s1], VZ[s2], VZ[s3], L[3], Rr[3]);
-        recon(mode, PR[s0], PR[s1], PR[s2], PR[s3], L[4], Rr[4]);
-        recon(mode, 0.5f * (BY[s0] + BY[s0 + LX]), 0.5f * (BY[s1] + BY[s1 + LX]),
-              0.5f * (BY[s2] + BY[s2 + LX]), 0.5f * (BY[s3] + BY[s3 + LX]), L[5], Rr[5]);
-        recon(mode, 0.5f * (BZ[s0] + BZN[s0]), 0.5f * (BZ[s1] + BZN[s1]),
-              0.5f * (BZ[s2] + BZN[s2]), 0.5f * (BZ[s3] + BZN[s3]), L[6], Rr[6]);
-        if (src) {
-          const float mbv = 0.5f * h * accel_x(b, 0.5f * (VY[s1] + VY[s2]));
-          L[1] += mbv;
-          Rr[1] += mbv;
+        for (int v = 0; v < 7; ++v) q[s][v] = q[s + 1][v];
+      q[3][0] = P0[c];
+      q[3][1] = P0[3 * LN + c];
+      q[3][2] = P0[1 * LN + c];
+      q[3][3] = P0[2 * LN + c];
+      q[3][4] = P0[4 * LN + c];
+      q[3][5] = 0.5f * (P0[5 * LN + c] + P0[5 * LN + c + 1]);
+      q[3][6] = 0.5f * (P0[6 * LN + c] + P0[6 * LN + c + LX]);
+      bz_prev = bz_cur;
+      bz_cur = P0[7 * LN + c];
+    }
+    const bool xy_plane = p >= z0 && p < zend;
+    if (xy_plane) {
+      const int yrow = y < ny ? y : y - ny;  // the face row y == ny is face 0
+      if (y < ny && x <= nx) {
+        const Flux7 r = face_x(cb, nb, c);
+        put_x(ty, tx, r);
+        if (valid) {
+          const int a = wrap(p, nz) * plane + y * nx + x;
+          SB.fb[0][a] = r.bt1;  // B_y
+          SB.fb[1][a] = r.bt2;  // B_z
         }
-        const Flux7 r = hlld(b.gamma, b.inv_gm1, BX[c], L[0], Rr[0], L[1], Rr[1], L[2], Rr[2],
-                             L[3], Rr[3], L[4], Rr[4], L[5], Rr[5], L[6], Rr[6]);
-        F.dir[0].f[0][a] = r.rho;
-        F.dir[0].f[1][a] = r.mn;
-        F.dir[0].f[2][a] = r.mt1;
-        F.dir[0].f[3][a] = r.mt2;
-        F.dir[0].f[4][a] = r.e;
-        F.dir[0].f[5][a] = r.bt1;  // B_y
-        F.dir[0].f[6][a] = r.bt2;  // B_z
       }
-      // ---- y-face
-      {
-        float L[7], Rr[7];
-        const int s0 = c - 2 * LX, s1 = c - LX, s2 = c, s3 = c + LX;
-        recon(mode, R[s0], R[s1], R[s2], R[s3], L[0], Rr[0]);
-        recon(mode, VY[s0], VY[s1], VY[s2], VY[s3], L[1], Rr[1]);
-        recon(mode, VZ[s0], VZ[s1], VZ[s2], VZ[s3], L[2], Rr[2]);
-        recon(mode, VX[s0], VX[s1], VX[s2], VX[s3], L[3], Rr[3]);
-        recon(mode, PR[s0], PR[s1], PR[s2], PR[s3], L[4], Rr[4]);
-        recon(mode, 0.5f * (BZ[s0] + BZN[s0]), 0.5f * (BZ[s1] + BZN[s1]),
-              0.5f * (BZ[s2] + BZN[s2]), 0.5f * (BZ[s3] + BZN[s3]), L[5], Rr[5]);
-        recon(mode, 0.5f * (BX[s0] + BX[s0 + 1]), 0.5f * (BX[s1] + BX[s1 + 1]),
-              0.5f * (BX[s2] + BX[s2 + 1]), 0.5f * (BX[s3] + BX[s3 + 1]), L[6], Rr[6]);
-        if (src) {
-          const float mbv = 0.5f * h * accel_y(b, 0.5f * (VX[s1] + VX[s2]), y);
-          L[1] += mbv;
-          Rr[1] += mbv;
+      if (x < nx && y <= ny) {
+        const Flux7 r = face_y(cb, nb, c, yrow);
+        put_y(ty, tx, r);
+        if (valid) {
+          const int a = wrap(p, nz) * plane + y * nx + x;
+          SB.fb[2][a] = r.bt1;  // B_z
+          SB.fb[3][a] = r.bt2;  // B_x
         }
-        const Flux7 r = hlld(b.gamma, b.inv_gm1, BY[c], L[0], Rr[0], L[1], Rr[1], L[2], Rr[2],
-                             L[3], Rr[3], L[4], Rr[4], L[5], Rr[5], L[6], Rr[6]);
-        F.dir[1].f[0][a] = r.rho;
-        F.dir[1].f[2][a] = r.mn;
-        F.dir[1].f[3][a] = r.mt1;
-        F.dir[1].f[1][a] = r.mt2;
-        F.dir[1].f[4][a] = r.e;
-        F.dir[1].f[5][a] = r.bt1;  // B_z
-        F.dir[1].f[6][a] = r.bt2;  // B_x
+      }
+      // the tile's upper boundary faces
+      if (tid < TY) {
+        const int yy = y0 + tid;
+        if (yy < ny && x0 + TX <= nx) put_x(tid, TX, face_x(cb, nb, (tid + 2) * LX + TX + 2));
+      } else if (tid < TY + TX) {
+        const int col = tid - TY, yy = y0 + TY;
+        if (x0 + col < nx && yy <= ny)
+          put_y(TY, col, face_y(cb, nb, (TY + 2) * LX + col + 2, yy < ny ? yy : yy - ny));
       }
     }
-    // ---- z-face of plane p-1 from the queue (planes p-3 .. p)
-    if (valid && p - 1 >= z0 && p - 1 < zend) {
-      const int a = wrap(p - 1, nz) * plane + y * nx + x;
+    // z-face below plane p-1 (one plane beyond the chunk: the upper face of
+    // its last plane)
+    float fzn[5] = {};
+    if (valid && p - 1 >= z0 && p - 1 <= zend) {
       float L[7], Rr[7];
 #pragma unroll
       for (int v = 0; v < 7; ++v) recon(mode, q[0][v], q[1][v], q[2][v], q[3][v], L[v], Rr[v]);
       // the z acceleration is zero: the face predictor adds nothing
       const Flux7 r = hlld(b.gamma, b.inv_gm1, bz_prev, L[0], Rr[0], L[1], Rr[1], L[2], Rr[2],
                            L[3], Rr[3], L[4], Rr[4], L[5], Rr[5], L[6], Rr[6]);
-      F.dir[2].f[0][a] = r.rho;
-      F.dir[2].f[3][a] = r.mn;
-      F.dir[2].f[1][a] = r.mt1;
-      F.dir[2].f[2][a] = r.mt2;
-      F.dir[2].f[4][a] = r.e;
-      F.dir[2].f[5][a] = r.bt1;  // B_x
-      F.dir[2].f[6][a] = r.bt2;  // B_y
+      fzn[0] = r.rho;
+      fzn[1] = r.mt1;
+      fzn[2] = r.mt2;
+      fzn[3] = r.mn;
+      fzn[4] = r.e;
+      if (p - 1 < zend) {
+        const int a = wrap(p - 1, nz) * plane + y * nx + x;
+        SB.fb[4][a] = r.bt1;  // B_x
+        SB.fb[5][a] = r.bt2;  // B_y
+      }
     }
     __syncthreads();
+    // x + y divergence of plane p (X, Y order)
+    float pxy[5] = {};
+    if (valid && xy_plane) {
+      constexpr int sx = TY * (TX + 1), sy = (TY + 1) * TX;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) {
+        const float *fx = FXL + k * sx + ty * (TX + 1) + tx;
+        const float *fy = FYL + k * sy + ty * TX + tx;
+        float sacc = (fx[0] - fx[1]) * idx;
+        sacc += (fy[0] - fy[TX]) * idy;
+        pxy[k] = sacc;
+      }
+    }
+    // plane p-2 complete: its lower z-face (fzl) and upper z-face (fzn)
+    if (valid && p - 2 >= z0 && p - 2 < zend) {
+      const int a = wrap(p - 2, nz) * plane + y * nx + x;
+#pragma unroll
+      for (int k = 0; k < 5; ++k) SB.pace[k][a] = pxy2[k] + (fzl[k] - fzn[k]) * idz;
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      pxy2[k] = pxy1[k];
+      pxy1[k] = pxy[k];
+      fzl[k] = fzn[k];
+    }
     if (prefetch) store_plane(cb);  // plane p is no longer needed
     __syncthreads();
   }
@@ -263,29 +373,29 @@ struct UpdateShape {
   static_assert(TX * TY == kTileThreads, "tiles have 256 threads");
   static constexpr int RX = TX + 2, RN = RX * (TY + 2);
   static constexpr int NR = (RN + kTileThreads - 1) / kTileThreads;  // ring slots per thread
-  // LDS layout (floats): S3[5][3][RN] | SZ[2][2][RN] | SP[2][14][RN]
+  // LDS layout (floats): S3[5][3][RN] | SZ[2][2][RN] | SP[4][RN]
   static constexpr int kS3 = 0, kSZ = 15 * RN, kSP = 19 * RN;
-  static constexpr int kLds = 47 * RN;
+  static constexpr int kLds = 23 * RN;
 };
 
 // The cell update as a z march: a TX x TY column tile plus a one-cell ring
-// is staged in LDS rings, one plane ahead (prefetched into registers while
-// the current plane is computed), so every neighbour access of the EMFs and
-// the divergence is an LDS read and each HBM value is fetched once per tile.
-//   S3 (planes r-1, r, r+1): v_x, v_y, v_z, the x-flux of B_z, the y-flux of B_z;
-//   SZ (faces r, r+1): the z-fluxes of B_x and B_y;
-//   SP (plane r): the hydro x/y fluxes, the x-flux of B_y, the y-flux of B_x
-//   and the base B_x, B_y (for the upper faces).
-// The z-flux hydro components and the base cell values are the thread's own
-// and stay in registers.  cfl_t is lowered to the CFL time of every interior
+// is staged in LDS, one plane ahead (prefetched into registers while the
+// current plane is computed), so every neighbour access of the EMFs is an
+// LDS read and each HBM value is fetched once per tile.
+//   S3 (planes r-1, r, r+1): v_x, v_y, v_z, x-face F(B_z), y-face F(B_z);
+//   SZ (faces r, r+1): z-face F(B_x), F(B_y);
+//   SP (plane r): x-face F(B_y), y-face F(B_x), base B_x, B_y (upper faces).
+// The hydro divergence and the base cell values are the thread's own and
+// stay in registers.  cfl_t is lowered to the CFL time of every interior
 // cell the thread updates (stage 2 with Cfl).
 template <int TX, int TY, bool PaceOnly, bool Cfl>
-__device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, const FluxSet &F,
+__device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, const FluxSet &Fs,
                                             const FieldPtrs &Wb, const FieldPtrs &Wo, float h,
                                             int x0, int y0, int z0, int zend, float *lds,
                                             float &cfl_t) {
   using S = UpdateShape<TX, TY>;
   constexpr int NT = kTileThreads, RX = S::RX, RN = S::RN, NR = S::NR;
+  const StageBuf SB = stage_buf(Fs);
   float *const S3 = lds + S::kS3;
   float *const SZ = lds + S::kSZ;
   float *const SP = lds + S::kSP;
@@ -302,14 +412,11 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
     const int li = tid + s * NT;
     gxy[s] = (li < RN) ? pmod(y0 - 1 + li / RX, ny) * nx + pmod(x0 - 1 + li % RX, nx) : 0;
   }
-  const float *a3[5] = {Win.f[1], Win.f[2], Win.f[3], F.dir[0].f[6], F.dir[1].f[5]};
-  const float *az[2] = {F.dir[2].f[5], F.dir[2].f[6]};
-  const float *ap[14] = {F.dir[0].f[0], F.dir[0].f[1], F.dir[0].f[2], F.dir[0].f[3],
-                         F.dir[0].f[4], F.dir[0].f[5], F.dir[1].f[0], F.dir[1].f[1],
-                         F.dir[1].f[2], F.dir[1].f[3], F.dir[1].f[4], F.dir[1].f[6],
-                         Wb.f[5],       Wb.f[6]};
+  const float *a3[5] = {Win.f[1], Win.f[2], Win.f[3], SB.fb[1], SB.fb[2]};
+  const float *az[2] = {SB.fb[4], SB.fb[5]};
+  const float *ap[4] = {SB.fb[0], SB.fb[3], Wb.f[5], Wb.f[6]};
   // staging registers
-  float r3[NR][5], rz[NR][2], rp[NR][14];
+  float r3[NR][5], rz[NR][2], rp[NR][4];
   auto load3 = [&](int p) {
     const int base = wrap(p, nz) * plane;
 #pragma unroll
@@ -347,30 +454,27 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
     for (int s = 0; s < NR; ++s)
       if (tid + s * NT < RN)
 #pragma unroll
-        for (int k = 0; k < 14; ++k) rp[s][k] = ap[k][base + gxy[s]];
+        for (int k = 0; k < 4; ++k) rp[s][k] = ap[k][base + gxy[s]];
   };
-  auto storep = [&](int p) {
+  auto storep = [&]() {
 #pragma unroll
     for (int s = 0; s < NR; ++s)
       if (tid + s * NT < RN)
 #pragma unroll
-        for (int k = 0; k < 14; ++k) SP[((p & 1) * 14 + k) * RN + tid + s * NT] = rp[s][k];
+        for (int k = 0; k < 4; ++k) SP[k * RN + tid + s * NT] = rp[s][k];
   };
-  // own-column registers: z-flux hydro at faces r, r+1 and base cell values
+  // own-column registers: hydro divergence and base cell values of plane r
   const int own = valid ? y * nx + x : 0;
-  float fz_lo[5], fz_hi[5], fz_nx[5], wb[8], wb_nx[8], bz_hi = 0.0f, bz_nx = 0.0f;
-  auto load_own_fz = [&](int p, float *dst) {
+  float pc[5], pc_nx[5], wb[8], wb_nx[8], bz_hi = 0.0f, bz_nx = 0.0f;
+  auto load_own = [&](int p, float *dpc, float *dwb) {
     const int base = wrap(p, nz) * plane + own;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) dst[k] = F.dir[2].f[k][base];
-  };
-  auto load_own_w = [&](int p, float *dst) {
-    const int base = wrap(p, nz) * plane + own;
+    for (int k = 0; k < 5; ++k) dpc[k] = SB.pace[k][base];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) dst[k] = Wb.f[k][base];
+    for (int k = 0; k < 8; ++k) dwb[k] = Wb.f[k][base];
   };
 
-  // prologue: planes z0-1, z0, z0+1 of the 3-rings, faces z0, z0+1, plane z0
+  // prologue: planes z0-1, z0, z0+1 of the 3-ring, faces z0, z0+1, plane z0
   load3(z0 - 1);
   store3(z0 - 1);
   load3(z0);
@@ -382,10 +486,8 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
   loadz(z0 + 1);
   storez(z0 + 1);
   loadp(z0);
-  storep(z0);
-  load_own_fz(z0, fz_lo);
-  load_own_fz(z0 + 1, fz_hi);
-  load_own_w(z0, wb);
+  storep();
+  load_own(z0, pc, wb);
   bz_hi = Wb.f[7][wrap(z0 + 1, nz) * plane + own];
   __syncthreads();
 
@@ -398,13 +500,11 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
       load3(r + 2);
       loadz(r + 2);
       loadp(r + 1);
-      load_own_fz(r + 2, fz_nx);
-      load_own_w(r + 1, wb_nx);
+      load_own(r + 1, pc_nx, wb_nx);
       bz_nx = Wb.f[7][wrap(r + 2, nz) * plane + own];
     }
     const int sm = (r + 2) % 3, s0 = (r + 3) % 3, sp = (r + 4) % 3;  // planes r-1, r, r+1
     const int f0 = r & 1, f1 = (r + 1) & 1;                         // faces r, r+1
-    const int pr_ = r & 1;                                          // plane r
     const float *VX[3] = {S3 + (0 * 3 + sm) * RN, S3 + (0 * 3 + s0) * RN, S3 + (0 * 3 + sp) * RN};
     const float *VY[3] = {S3 + (1 * 3 + sm) * RN, S3 + (1 * 3 + s0) * RN, S3 + (1 * 3 + sp) * RN};
     const float *VZ[3] = {S3 + (2 * 3 + sm) * RN, S3 + (2 * 3 + s0) * RN, S3 + (2 * 3 + sp) * RN};
@@ -414,25 +514,10 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
                            S3 + (4 * 3 + sp) * RN};
     const float *FZ5[2] = {SZ + (0 * 2 + f0) * RN, SZ + (0 * 2 + f1) * RN};
     const float *FZ6[2] = {SZ + (1 * 2 + f0) * RN, SZ + (1 * 2 + f1) * RN};
-    const float *FXk[6], *FYk[6];
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-      FXk[k] = SP + (pr_ * 14 + k) * RN;
-      FYk[k] = SP + (pr_ * 14 + 6 + k) * RN;
-    }
-    const float *BXB = SP + (pr_ * 14 + 12) * RN, *BYB = SP + (pr_ * 14 + 13) * RN;
+    const float *FX5 = SP, *FY6 = SP + RN, *BXB = SP + 2 * RN, *BYB = SP + 3 * RN;
 
     if (valid) {
       const int a = r * plane + own;
-      // divergence (X, Y, Z order)
-      float Pc[5];
-#pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        float sacc = (FXk[k][c] - FXk[k][c + 1]) * idx;
-        sacc += (FYk[k][c] - FYk[k][c + RX]) * idy;
-        sacc += (fz_lo[k] - fz_hi[k]) * idz;
-        Pc[k] = sacc;
-      }
       // Edge EMFs; u (0: lower, 1: upper) indexes the edge along each axis.
       // EXY[u][v]: x-flux of B_y at x-face i+u, upwinded in y across y-face j+v
       // EYX[u][v]: y-flux of B_x at y-face j+v, upwinded in x across x-face i+u
@@ -447,8 +532,8 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
           const int e = c + u + v * RX;  // cell (i+u, j+v)
           const float sy = VY[1][e] + VY[1][e - RX] + VY[1][e - 1] + VY[1][e - 1 - RX];
           const float sx = VX[1][e] + VX[1][e - 1] + VX[1][e - RX] + VX[1][e - 1 - RX];
-          EXY[u][v] = upwind(sy, FXk[5][e - RX], FXk[5][e]);
-          EYX[u][v] = upwind(sx, FYk[5][e - 1], FYk[5][e]);
+          EXY[u][v] = upwind(sy, FX5[e - RX], FX5[e]);
+          EYX[u][v] = upwind(sx, FY6[e - 1], FY6[e]);
           // Y-edge at x-face i+u, z-face r+v: cells x in {i+u-1, i+u}, planes {r+v-1, r+v}
           const int ex = c + u;
           const float sz = VZ[v + 1][ex] + VZ[v][ex] + VZ[v + 1][ex - 1] + VZ[v][ex - 1];
@@ -462,20 +547,21 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
           EYZ[u][v] = upwind(sz2, FY5[v][ey], FY5[v + 1][ey]);
           EZY[u][v] = upwind(sy2, FZ6[v][ey - RX], FZ6[v][ey]);
         }
+      const float idx_ = idx, idy_ = idy, idz_ = idz;
       float pbx[2], pby[2], pbz[2];
 #pragma unroll
       for (int u = 0; u < 2; ++u) {
-        pbx[u] = 0.5f * ((EXY[u][1] - EXY[u][0]) * idy + (EXZ[u][1] - EXZ[u][0]) * idz) -
-                 0.5f * (EYX[u][1] - EYX[u][0]) * idy - 0.5f * (EZX[u][1] - EZX[u][0]) * idz;
-        pby[u] = -0.5f * (EXY[1][u] - EXY[0][u]) * idx +
-                 0.5f * ((EYZ[u][1] - EYZ[u][0]) * idz + (EYX[1][u] - EYX[0][u]) * idx) -
-                 0.5f * (EZY[u][1] - EZY[u][0]) * idz;
-        pbz[u] = -0.5f * (EXZ[1][u] - EXZ[0][u]) * idx - 0.5f * (EYZ[1][u] - EYZ[0][u]) * idy +
-                 0.5f * ((EZX[1][u] - EZX[0][u]) * idx + (EZY[1][u] - EZY[0][u]) * idy);
+        pbx[u] = 0.5f * ((EXY[u][1] - EXY[u][0]) * idy_ + (EXZ[u][1] - EXZ[u][0]) * idz_) -
+                 0.5f * (EYX[u][1] - EYX[u][0]) * idy_ - 0.5f * (EZX[u][1] - EZX[u][0]) * idz_;
+        pby[u] = -0.5f * (EXY[1][u] - EXY[0][u]) * idx_ +
+                 0.5f * ((EYZ[u][1] - EYZ[u][0]) * idz_ + (EYX[1][u] - EYX[0][u]) * idx_) -
+                 0.5f * (EZY[u][1] - EZY[u][0]) * idz_;
+        pbz[u] = -0.5f * (EXZ[1][u] - EXZ[0][u]) * idx_ - 0.5f * (EYZ[1][u] - EYZ[0][u]) * idy_ +
+                 0.5f * ((EZX[1][u] - EZX[0][u]) * idx_ + (EZY[1][u] - EZY[0][u]) * idy_);
       }
       if (PaceOnly) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) Wo.f[k][a] = Pc[k];
+        for (int k = 0; k < 5; ++k) Wo.f[k][a] = pc[k];
         Wo.f[5][a] = pbx[0];
         Wo.f[6][a] = pby[0];
         Wo.f[7][a] = pbz[0];
@@ -487,11 +573,11 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
         const float e0 = wb[4] * b.inv_gm1 + 0.5f * rho0 * (vx0 * vx0 + vy0 * vy0 + vz0 * vz0) +
                          0.25f * (bxl0 * bxl0 + bxh0 * bxh0 + byl0 * byl0 + byh0 * byh0 +
                                   bzl0 * bzl0 + bzh0 * bzh0);
-        const float rho = rho0 + h * Pc[0];
-        const float mx = rho0 * vx0 + h * Pc[1];
-        const float my = rho0 * vy0 + h * Pc[2];
-        const float mz = rho0 * vz0 + h * Pc[3];
-        const float e = e0 + h * Pc[4];
+        const float rho = rho0 + h * pc[0];
+        const float mx = rho0 * vx0 + h * pc[1];
+        const float my = rho0 * vy0 + h * pc[2];
+        const float mz = rho0 * vz0 + h * pc[3];
+        const float e = e0 + h * pc[4];
         const float bxl = bxl0 + h * pbx[0], bxh = bxh0 + h * pbx[1];
         const float byl = byl0 + h * pby[0], byh = byh0 + h * pby[1];
         const float bzl = bzl0 + h * pbz[0], bzh = bzh0 + h * pbz[1];
@@ -541,12 +627,9 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
     if (more) {
       store3(r + 2);
       storez(r + 2);
-      storep(r + 1);
+      storep();
 #pragma unroll
-      for (int k = 0; k < 5; ++k) {
-        fz_lo[k] = fz_hi[k];
-        fz_hi[k] = fz_nx[k];
-      }
+      for (int k = 0; k < 5; ++k) pc[k] = pc_nx[k];
 #pragma unroll
       for (int k = 0; k < 8; ++k) wb[k] = wb_nx[k];
       bz_hi = bz_nx;
