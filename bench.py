@@ -67,7 +67,12 @@ def parse():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--size", type=int, nargs=3, default=(256, 256, 256))
+    ap.add_argument("--size", type=int, nargs=3, default=None,
+                    help="interior cells (default 256^3; 512 1024 512 for --workload large)")
+    ap.add_argument("--workload", choices=("mri", "large"), default="mri",
+                    help="mri: the 256^3 beta-400 shearing box with velocity noise; large: "
+                         "the 512x1024x512 beta-1000 box with the filtered density noise "
+                         "(per-stage GB/s; no CPU baseline, batched or adjoint legs)")
     ap.add_argument("--path", choices=("auto", "unfused", "tiled", "strict"), default="auto")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -263,6 +268,29 @@ def adjoint_leg(a, torch):
     }
 
 
+def stage_rates(cfg, tim, kbytes, zones):
+    """Algorithmic GB/s of each stage of the step: reconstruction + HLLD (the
+    face kernels), CT + update (the update kernels), CFL reduction (the
+    ghost-cell pass; the interior is reduced in the update's epilogue) and the
+    boundary (shearing-periodic ghost fill, 8 fields read + written per ghost
+    cell)."""
+    m = cfg["m"][1]
+    ghost_cells = 2 * m * cfg["n"][0] * cfg["n"][2]
+
+    def rate(names, nbytes):
+        ms = sum(tim[k][0] for k in names if k in tim)
+        n = sum(tim[k][1] for k in names if k in tim)
+        return None if not n else nbytes * n / (ms * 1e-3) / 1e9
+
+    return {
+        "reconstruct_hlld": rate(["faces_s1", "faces_s2"], kbytes["faces_s1"] * zones),
+        "ct_update": rate(["update_s1", "update_s2"],
+                          0.5 * (kbytes["update_s1"] + kbytes["update_s2"]) * zones),
+        "cfl_reduce": rate(["cfl_ghosts"], 32.0 * ghost_cells),
+        "boundary": rate(["ghost_y"], 64.0 * ghost_cells),
+    }
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -282,12 +310,25 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    large = a.workload == "large"
+    if a.size is None:
+        a.size = (512, 1024, 512) if large else (256, 256, 256)
+    if large:
+        a.no_cpu = a.no_batched = a.no_adjoint = True
     cfg = P.config_c2(tuple(a.size))
-    W = P.shearing_box_mri(cfg, seed=1002)
     path = {"auto": capi.PATH_AUTO, "unfused": capi.PATH_UNFUSED,
             "tiled": capi.PATH_TILED, "strict": capi.PATH_STRICT}[a.path]
     ctx = capi.Context(cfg, path=path)
-    ctx.set_state(W)
+    if large:
+        Wd = P.shearing_box_noise_device(cfg, seed=1003, beta=1000.0)
+        torch.cuda.synchronize()
+        ctx.set_state_device([Wd[i].data_ptr() for i in range(8)])
+        del Wd
+        torch.cuda.empty_cache()
+        W = None
+    else:
+        W = P.shearing_box_mri(cfg, seed=1002)
+        ctx.set_state(W)
     goal = 1e30
     zones = P.interior_cells(cfg)
 
@@ -336,8 +377,11 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded shearing-box MRI initial state)",
         "config": {"workload": "shearing-box MRI step, %dx%dx%d cells (array %dx%dx%d incl. "
-                               "4 radial ghost layers/side), gamma 5/3, beta 400, CFL 0.4"
-                               % (tuple(cfg["global_size"]) + tuple(cfg["n"])),
+                               "4 radial ghost layers/side), gamma 5/3, %s, CFL 0.4"
+                               % (tuple(cfg["global_size"]) + tuple(cfg["n"]) +
+                                  ("beta 1000, filtered density noise 1e-3 peaked at k=8 "
+                                   "(seed 1003)" if large else
+                                   "beta 400, velocity noise 0.01 (seed 1002)",)),
                    "global_size": cfg["global_size"], "steps": a.steps,
                    "path": path_name},
         "roofline": roofline,
@@ -346,6 +390,8 @@ def main():
                           "bytes_per_zone_update": STEP_BYTES},
         "kernel_ms_per_step": {k: v[0] / a.profile_steps for k, v in tim.items() if v[1]},
     }
+    if large:
+        out["stage_gbps"] = stage_rates(cfg, tim, kbytes, zones)
     ctx.close()
     out["batched"] = None if a.no_batched else batched_leg(a, torch)
     out["adjoint"] = None if a.no_adjoint else adjoint_leg(a, torch)

@@ -183,6 +183,19 @@ int nmhd_synchronize(nmhd_ctx *ctx, int *steps_taken, int *goal_reached);
 int nmhd_hlld_batch(size_t n, float gamma, int path, const float *const in[15],
                     float *const out[7]);
 
+/* Run diagnostics reduced on the device (no array leaves the GPU): the
+ * quantities of the Kepler-disk monitor (application/kepler_disk_demo/
+ * mhd_solver.curb:107-130, which copies B_x and B_y to the host each step)
+ * and conservation checks.  Synchronises. */
+typedef struct nmhd_diagnostics {
+  double b2_mean[3]; /* <B_d^2> over every face of the array (ghosts included),
+                        squares in FP32 summed in FP64 as the monitor does */
+  double total[8];   /* interior sums of rho, m_x, m_y, m_z, E (with the
+                        face-averaged B^2), B_x, B_y, B_z */
+  double divb_max;   /* max over the interior of |div B| * min(d) */
+} nmhd_diagnostics;
+int nmhd_get_diagnostics(nmhd_ctx *ctx, nmhd_diagnostics *out);
+
 /* Per-kernel timing with HIP events on the context's stream. */
 int nmhd_timing_enable(nmhd_ctx *ctx, int enable);
 int nmhd_timing_reset(nmhd_ctx *ctx);

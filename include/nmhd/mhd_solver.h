@@ -176,14 +176,25 @@ class LocalSetup {
   Real computational_corner_[3] = {0, 0, 0};
   Real computational_extent_[3] = {0, 0, 0};
   Real d_[3] = {0, 0, 0};
+  std::ofstream ofs_log_, ofs_bench_;
 
   LocalSetup() {}
+  // opens <dir>/log/<rank>.txt and <dir>/bench/<rank>.txt for appending, as
+  // the reference's constructor does (mhd_solver.hrb:81-88, 133-138)
   LocalSetup(int mpi_rank0, std::string output_dir0) : mpi_rank_(mpi_rank0) {
     char work[32];
     std::snprintf(work, sizeof work, "%04d", mpi_rank0);
     output_directory_ = output_dir0;
     mpi_prefix_ = work;
+    ofs_log_.open(log_filename().c_str(), std::ios_base::out | std::ios_base::app);
+    ofs_bench_.open(bench_filename().c_str(), std::ios_base::out | std::ios_base::app);
   }
+  LocalSetup(const LocalSetup &) = delete;
+  LocalSetup &operator=(const LocalSetup &) = delete;
+  std::ofstream &ofs_log() { return ofs_log_; }
+  std::ofstream &ofs_bench() { return ofs_bench_; }
+  std::string log_filename() const { return mkdir_prefix("log") + ".txt"; }
+  std::string bench_filename() const { return mkdir_prefix("bench") + ".txt"; }
   int &mpi_rank() { return mpi_rank_; }
   std::string &output_directory() { return output_directory_; }
   const std::string &output_directory() const { return output_directory_; }
@@ -427,6 +438,15 @@ class MHDSolver {
   }
 
   int monitor() { return monitor_hook ? monitor_hook(*this) : 0; }
+
+  // Run diagnostics reduced on the GPU: <B_d^2> over every face (what the
+  // Kepler-disk monitor computes on the host, application/kepler_disk_demo/
+  // mhd_solver.curb:107-130), conservation totals and max |div B|.
+  nmhd_diagnostics diagnostics() {
+    nmhd_diagnostics d;
+    check(nmhd_get_diagnostics(ctx_, &d), "nmhd_get_diagnostics");
+    return d;
+  }
   int initial_event() { return initial_event_hook ? initial_event_hook(*this) : 0; }
   int final_event() { return final_event_hook ? final_event_hook(*this) : 0; }
 

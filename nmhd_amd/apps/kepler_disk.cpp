@@ -67,7 +67,13 @@ int main(int argc, char **argv) {
     B[Z][o - HZ] = b0;
   };
   int generation_limit = max_steps;
+  // the reference's monitor (mhd_solver.curb:107-130) with the averages
+  // reduced on the GPU instead of copying B_x, B_y to the host
   solver.monitor_hook = [&](MHDSolver &s) {
+    const nmhd_diagnostics d = s.diagnostics();
+    s.local_setup().ofs_log() << "t: " << s.current_time() << " B_init: " << b0
+                              << " Bx_squared_average: " << d.b2_mean[0]
+                              << " By_squared_average: " << d.b2_mean[1] << std::endl;
     return (generation_limit >= 0 && s.generation() >= generation_limit) ? 1 : 0;
   };
   if (fresh) {

@@ -24,7 +24,7 @@ EXPORTS = (
     "nmhd_step", "nmhd_run", "nmhd_reset_run", "nmhd_enqueue_steps",
     "nmhd_synchronize", "nmhd_hlld_batch", "nmhd_timing_enable",
     "nmhd_timing_reset", "nmhd_timing_get", "nmhd_set_path", "nmhd_get_path",
-    "nmhd_set_tiles", "nmhd_get_tiles", "nmhd_batch_create", "nmhd_batch_destroy", "nmhd_batch_last_error",
+    "nmhd_set_tiles", "nmhd_get_tiles", "nmhd_get_diagnostics", "nmhd_batch_create", "nmhd_batch_destroy", "nmhd_batch_last_error",
     "nmhd_batch_get_geometry", "nmhd_batch_set_state", "nmhd_batch_get_state",
     "nmhd_batch_set_time", "nmhd_batch_get_time", "nmhd_batch_get_status",
     "nmhd_batch_set_tiles", "nmhd_batch_set_launch", "nmhd_batch_get_launch",
@@ -50,6 +50,14 @@ class NmhdDesc(ctypes.Structure):
         ("recon", ctypes.c_int),
         ("integrator", ctypes.c_int),
         ("path", ctypes.c_int),
+    ]
+
+
+class NmhdDiagnostics(ctypes.Structure):
+    _fields_ = [
+        ("b2_mean", ctypes.c_double * 3),
+        ("total", ctypes.c_double * 8),
+        ("divb_max", ctypes.c_double),
     ]
 
 
@@ -109,6 +117,7 @@ def load():
         "nmhd_set_path": [_VP, ctypes.c_int],
         "nmhd_get_path": [_VP, _IP],
         "nmhd_set_tiles": [_VP, ctypes.c_int, ctypes.c_int],
+        "nmhd_get_diagnostics": [_VP, ctypes.POINTER(NmhdDiagnostics)],
         "nmhd_get_tiles": [_VP, _IP, _IP],
         "nmhd_batch_create": [ctypes.POINTER(NmhdDesc), ctypes.c_int, ctypes.POINTER(_VP)],
         "nmhd_batch_destroy": [_VP],
@@ -279,6 +288,12 @@ class Context:
         p = ctypes.c_int()
         self._c(load().nmhd_get_path(self.h, ctypes.byref(p)))
         return p.value
+
+    def diagnostics(self):
+        """{'b2_mean': [3], 'total': [8], 'divb_max': float} reduced on the GPU."""
+        d = NmhdDiagnostics()
+        self._c(load().nmhd_get_diagnostics(self.h, ctypes.byref(d)))
+        return {"b2_mean": list(d.b2_mean), "total": list(d.total), "divb_max": d.divb_max}
 
     def set_tiles(self, faces_tile, update_tile):
         self._c(load().nmhd_set_tiles(self.h, faces_tile, update_tile))

@@ -124,6 +124,8 @@ struct nmhd_ctx {
   float *scratch8[8] = {};
   StepState *st = nullptr;
   StepState *diag = nullptr;
+  double *diag_part = nullptr;  // run diagnostics: per-block partials, result
+  double *diag_out = nullptr;
   int path = NMHD_PATH_TILED;
   // tile shapes of the tiled path (fastest measured at 256^3 on MI355X)
   int ftile = TV_16x16, utile = TV_64x4;
@@ -301,6 +303,8 @@ void free_all(nmhd_ctx *c) {
     if (p) (void)hipFree(p), p = nullptr;
   if (c->st) (void)hipFree(c->st), c->st = nullptr;
   if (c->diag) (void)hipFree(c->diag), c->diag = nullptr;
+  if (c->diag_part) (void)hipFree(c->diag_part), c->diag_part = nullptr;
+  if (c->diag_out) (void)hipFree(c->diag_out), c->diag_out = nullptr;
   for (auto &p : c->pending) c->pool.push_back(p.a), c->pool.push_back(p.b);
   c->pending.clear();
   for (auto e : c->pool) (void)hipEventDestroy(e);
@@ -604,6 +608,24 @@ int nmhd_set_path(nmhd_ctx *c, int path) {
   if (!c || path < NMHD_PATH_AUTO || path > NMHD_PATH_STRICT) return NMHD_EINVAL;
   c->path = resolve_path(path);
   c->cfl_valid = false;
+  return NMHD_OK;
+}
+
+int nmhd_get_diagnostics(nmhd_ctx *c, nmhd_diagnostics *out) {
+  if (!c || !out) return NMHD_EINVAL;
+  if (!c->diag_part) {
+    if (hipMalloc(&c->diag_part, sizeof(double) * kDiagVals * diag_blocks(c->box)) != hipSuccess ||
+        hipMalloc(&c->diag_out, sizeof(double) * kDiagVals) != hipSuccess)
+      return fail(c, NMHD_ENOMEM, "diagnostics allocation failed");
+  }
+  launch_diag(c->stream, c->box, fields(c, c->cur), c->diag_part, c->diag_out);
+  double v[kDiagVals];
+  HIPCHECK(c, hipMemcpyAsync(v, c->diag_out, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+  int r = sync_and_check(c);
+  if (r) return r;
+  for (int d = 0; d < 3; ++d) out->b2_mean[d] = v[d] / (double)c->box.ncell;
+  for (int k = 0; k < 8; ++k) out->total[k] = v[3 + k];
+  out->divb_max = v[11];
   return NMHD_OK;
 }
 

@@ -167,6 +167,11 @@ void launch_batch_update(hipStream_t s, int grid, const BatchBox *boxes, const W
                          int nitems, float *state_in, float *flux, float *state_base,
                          float *state_out, StepState *st, int hsel, bool cfl);
 
+// kernels_diag.hip (run diagnostics: 12 FP64 values, see k_diag_partial)
+constexpr int kDiagVals = 12;
+int diag_blocks(const Box &b);
+void launch_diag(hipStream_t s, const Box &b, FieldPtrs W, double *part, double *out);
+
 // kernels_adjoint.hip (reverse of one RK stage of the periodic step)
 struct AdjStageArgs {
   FieldPtrs Win, Wbase, Wout;  // stage input, conserved base, stage output

@@ -307,3 +307,65 @@ def smooth_direction(cfg, seed, index, fields=(0, 1, 2, 3), nmodes=4):
         s = mode_sum(cfg, fourier_modes(seed, nmodes, stream=index * 8 + f))
         D[f] = s / max(np.sqrt((s * s).mean()), 1e-30)
     return D
+
+
+# ------------------------------------------------- large shearing box (C3)
+
+def _splitmix64_torch(x):
+    """splitmix64 on int64 tensors (two's-complement wrap, logical shifts)."""
+    import torch
+
+    def shr(v, k):
+        return (v >> k) & ((1 << (64 - k)) - 1)
+
+    def c(u):  # uint64 constant as a wrapped int64
+        return u - (1 << 64) if u >= (1 << 63) else u
+
+    z = x + c(0x9E3779B97F4A7C15)
+    z = (z ^ shr(z, 30)) * c(0xBF58476D1CE4E5B9)
+    z = (z ^ shr(z, 27)) * c(0x94D049BB133111EB)
+    return z ^ shr(z, 31)
+
+
+def shearing_box_noise_device(cfg, seed=1003, beta=1000.0, amp=1e-3, k0=8.0, width=2.0,
+                              rho0=1.0, p0=1.0, device="cuda"):
+    """The large MRI box on the device (torch tensors, (8, nz, ny, nx) fp32):
+    rho0 (1 + amp * n), n = white noise u in [-1, 1) from the counter RNG
+    (component 0), filtered in Fourier space by exp(-(|k|-k0)^2 / (2 width^2))
+    (|k| in units of the box's fundamental) and scaled to unit RMS; p = p0,
+    v_az = -q Omega x_rad, B = (0, 0, sqrt(2 p0 / beta)).  The noise lives on
+    the interior rows; ghost rows are filled by the first ghost fill."""
+    import torch
+    n = cfg["n"]
+    m = cfg["m"]
+    gx, gy, gz = cfg["global_size"]
+    W = torch.zeros((NFIELD, n[2], n[1], n[0]), dtype=torch.float32, device=device)
+    # white noise over the interior, one plane at a time (int64 counters)
+    noise = torch.empty((gz, gy, gx), dtype=torch.float32, device=device)
+    plane = gx * gy
+    base = torch.arange(plane, dtype=torch.int64, device=device)
+    for z in range(gz):
+        idx = base + z * plane
+        key = (idx * 8 + 0) ^ int(seed)
+        u = ((_splitmix64_torch(key) >> 40) & ((1 << 24) - 1)).to(torch.float64) * 2.0 ** -24
+        noise[z] = (2.0 * u - 1.0).to(torch.float32).view(gy, gx)
+    F = torch.fft.rfftn(noise)
+    kz = torch.fft.fftfreq(gz, d=1.0 / gz, device=device).view(-1, 1, 1)
+    ky = torch.fft.fftfreq(gy, d=1.0 / gy, device=device).view(1, -1, 1)
+    kx = torch.fft.rfftfreq(gx, d=1.0 / gx, device=device).view(1, 1, -1)
+    kk = torch.sqrt(kx * kx + ky * ky + kz * kz)
+    F *= torch.exp(-(kk - k0) ** 2 / (2.0 * width * width))
+    del kk
+    filt = torch.fft.irfftn(F, s=(gz, gy, gx))
+    del F
+    filt /= torch.sqrt((filt * filt).mean())
+    W[0, :, m[1]:m[1] + gy, :] = rho0 * (1.0 + amp * filt)
+    W[0, :, :m[1], :] = rho0
+    W[0, :, m[1] + gy:, :] = rho0
+    del filt, noise
+    W[4] = p0
+    corner_y = -cfg["ext"][1] / 2.0
+    yloc = torch.tensor(cell_centres(cfg, 1, corner_y), dtype=torch.float64, device=device)
+    W[1] = (-cfg["qshear"] * cfg["omega"] * yloc).to(torch.float32).view(1, -1, 1)
+    W[7] = math.sqrt(2.0 * p0 / beta)
+    return W

@@ -35,10 +35,11 @@ def counters(db):
     return {k: {c: sum(v) / len(v) for c, v in d.items()} for k, d in acc.items()}
 
 
-# rocprof kernel names -> per-launch slots of the tiled step
-SLOTS = (("k_faces<", ("faces_s1", "faces_s2")),
-         ("k_update_march<32, 8, false, true>", ("update_s2",)),
-         ("k_update_march<32, 8, false, false>", ("update_s1",)))
+# rocprof kernel names -> per-launch slots of the tiled step (the stage-2
+# update is the instantiation with the CFL epilogue: <TX, TY, false, true>)
+SLOTS = ((("k_faces<", ""), ("faces_s1", "faces_s2")),
+         (("k_update_march<", ", false, true>"), ("update_s2",)),
+         (("k_update_march<", ", false, false>"), ("update_s1",)))
 
 
 def traffic(fetch_db, write_db, fetch_scale, write_scale):
@@ -48,9 +49,9 @@ def traffic(fetch_db, write_db, fetch_scale, write_scale):
     f = counters(fetch_db)
     w = counters(write_db)
     out = {}
-    for pat, slots in SLOTS:
+    for (pat, suffix), slots in SLOTS:
         for k in f:
-            if pat in k and k in w:
+            if pat in k and k.endswith(suffix) and k in w:
                 fk = f[k]["FETCH_SIZE"]
                 wk = w[k]["WRITE_SIZE"]
                 rec = {"rocprof_kernel": k, "fetch_kib": fk, "write_kib": wk,

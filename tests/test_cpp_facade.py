@@ -54,3 +54,9 @@ def test_kepler_disk_app_continue(gpu_ready, tmp_path):
     assert r2.returncode == 0, r2.stdout + r2.stderr
     assert "generation = 5" in r2.stdout
     assert os.path.getsize(os.path.join(str(tmp_path), "c", "0000.bin")) == 12 + 8 * 4 * 16 * 24 * 16
+    # the monitor's log lines (reduced on the GPU), one per step of each run
+    log = open(os.path.join(str(tmp_path), "log", "0000.txt")).read().splitlines()
+    mon = [l for l in log if "Bx_squared_average:" in l]
+    assert len(mon) >= 5
+    by2 = [float(l.split("By_squared_average:")[1]) for l in mon]
+    assert all(v >= 0 for v in by2) and max(by2) < 1e-3
