@@ -15,7 +15,7 @@ using namespace nmhd;
 
 namespace {
 std::string g_adj_create_error;
-constexpr int kWork = 5 + 3 + 3 + 3 + 3 + 14;  // gU, gB, lE, lbb, abn, face
+constexpr int kWork = 5 + 3 + 3 + 3 + 3 + 42;  // gU, gB, lE, lbb, abn, face
 }  // namespace
 
 struct nmhd_adj {
@@ -94,7 +94,7 @@ AdjStageArgs stage_args(nmhd_adj *a) {
   for (int i = 0; i < 3; ++i) A.lE[i] = *w++;
   for (int i = 0; i < 3; ++i) A.lbb[i] = *w++;
   for (int i = 0; i < 3; ++i) A.abn[i] = *w++;
-  for (int i = 0; i < 14; ++i) A.face[i] = *w++;
+  for (int i = 0; i < 42; ++i) A.face[i] = *w++;
   return A;
 }
 

@@ -12,14 +12,15 @@
 //                 gB (3 face values per cell) and lam_base
 //   k_adj_edges   adjoint of the 6 CT edge-EMF families (3 distinct values per
 //                 cell: lamE_x, lamE_y, lamE_z)
-//   k_adj_face<D> per face: adjoint of the 7 fluxes (divergence + upwinded
-//                 EMFs, with the forward's upwind choice recomputed), then the
-//                 HLLD reverse sweep (adjoint_hlld.h) -> adjoints of the 7
-//                 left/right face states and of the normal field
-//   k_adj_gather<D> per cell: reverse of the minmod reconstruction from the 4
-//                 faces whose stencils hold the cell (forward minmod branches
-//                 recomputed) -> lam_in of rho, v, p and of the cell-averaged
-//                 transverse B
+//   k_adj_faces   per face of the three directions (blockIdx.y): adjoint of
+//                 the 7 fluxes (divergence + upwinded EMFs, with the forward's
+//                 upwind choice recomputed), then the HLLD reverse sweep
+//                 (adjoint_hlld.h) -> adjoints of the 7 left/right face states
+//                 and of the normal field
+//   k_adj_gather  per cell: reverse of the minmod reconstruction of the three
+//                 directions from the 4 faces whose stencils hold the cell
+//                 (forward minmod branches recomputed) -> lam_in of rho, v, p
+//                 and of the cell-averaged B
 //   k_adj_bfinish face B: cell-average and normal-field adjoints -> lam_in B
 #include "adjoint_hlld.h"
 #include "nmhd_kernels.h"
@@ -156,14 +157,11 @@ struct AdjFaceOut {
 };
 
 template <int D>
-__global__ void __launch_bounds__(256) k_adj_face(Box b, FieldPtrs W, const float *gU0,
-                                                  const float *gU1, const float *gU2,
-                                                  const float *gU3, const float *gU4,
-                                                  const float *lEx, const float *lEy,
-                                                  const float *lEz, float h, AdjFaceOut out) {
-  int x, y, z;
-  long a;
-  if (!cell_of(b, x, y, z, a)) return;
+__device__ __forceinline__ void adj_face(const Box &b, const FieldPtrs &W, const float *gU0,
+                                         const float *gU1, const float *gU2, const float *gU3,
+                                         const float *gU4, const float *lEx, const float *lEy,
+                                         const float *lEz, float h, const AdjFaceOut &out,
+                                         int x, int y, int z, long a) {
   constexpr int D1 = (D + 1) % 3, D2 = (D + 2) % 3;
   const float *gU[5] = {gU0, gU1, gU2, gU3, gU4};
   const long m1 = nb(b, x, y, z, D, -1);
@@ -264,6 +262,28 @@ __global__ void __launch_bounds__(256) k_adj_face(Box b, FieldPtrs W, const floa
   out.abn[a] = g.bx;
 }
 
+// All faces of the three directions in one launch: blockIdx.y is the
+// direction (a uniform branch per workgroup).
+struct AdjFaceOut3 {
+  AdjFaceOut d[3];
+};
+
+__global__ void __launch_bounds__(256) k_adj_faces(Box b, FieldPtrs W, const float *gU0,
+                                                   const float *gU1, const float *gU2,
+                                                   const float *gU3, const float *gU4,
+                                                   const float *lEx, const float *lEy,
+                                                   const float *lEz, float h, AdjFaceOut3 out) {
+  int x, y, z;
+  long a;
+  if (!cell_of(b, x, y, z, a)) return;
+  if (blockIdx.y == 0)
+    adj_face<0>(b, W, gU0, gU1, gU2, gU3, gU4, lEx, lEy, lEz, h, out.d[0], x, y, z, a);
+  else if (blockIdx.y == 1)
+    adj_face<1>(b, W, gU0, gU1, gU2, gU3, gU4, lEx, lEy, lEz, h, out.d[1], x, y, z, a);
+  else
+    adj_face<2>(b, W, gU0, gU1, gU2, gU3, gU4, lEx, lEy, lEz, h, out.d[2], x, y, z, a);
+}
+
 // ------------------------------------------------------------ k_adj_gather
 //
 // Cell c, variable x along D: L(face c+1) = x_c + s_c/2, R(face c) =
@@ -275,13 +295,11 @@ __device__ __forceinline__ void mm_grad(float a, float b, float &da, float &db) 
   db = (nz && !pick_a) ? 1.0f : 0.0f;
 }
 
+// The reconstruction reverse of direction D at cell (x,y,z): res[0..6] =
+// adjoints of rho, v_D, v_D+1, v_D+2, p, Bbar_D+1, Bbar_D+2 of the cell.
 template <int D>
-__global__ void __launch_bounds__(256) k_adj_gather(Box b, FieldPtrs W, AdjFaceOut fa,
-                                                    FieldPtrs lam_in, int first,
-                                                    float *lbb0, float *lbb1, float *lbb2) {
-  int x, y, z;
-  long a;
-  if (!cell_of(b, x, y, z, a)) return;
+__device__ __forceinline__ void adj_gather(const Box &b, const FieldPtrs &W, const AdjFaceOut &fa,
+                                           int x, int y, int z, float (&res)[7]) {
   constexpr int D1 = (D + 1) % 3, D2 = (D + 2) % 3;
   // cells c-2..c+2 and faces c-1..c+2 along D
   long cs[5];
@@ -290,7 +308,6 @@ __global__ void __launch_bounds__(256) k_adj_gather(Box b, FieldPtrs W, AdjFaceO
     int cx = (int)(c % b.n[0]), cy = (int)((c / b.n[0]) % b.n[1]), cz = (int)(c / b.n[0] / b.n[1]);
     return 0.5f * (W.f[5 + t][c] + W.f[5 + t][nb(b, cx, cy, cz, t, 1)]);
   };
-  float res[7];
   for (int k = 0; k < 7; ++k) {
     float xv[5];
     for (int s = 0; s < 5; ++s) {
@@ -320,30 +337,33 @@ __global__ void __launch_bounds__(256) k_adj_gather(Box b, FieldPtrs W, AdjFaceO
     }
     res[k] = r;
   }
-  // rho, v, p into lam_in (velocity components permuted back)
-  const int vf[3] = {1 + D, 1 + D1, 1 + D2};
-  if (first) {
-    lam_in.f[0][a] = res[0];
-    for (int i = 0; i < 3; ++i) lam_in.f[vf[i]][a] = res[1 + i];
-    lam_in.f[4][a] = res[4];
-  } else {
-    lam_in.f[0][a] += res[0];
-    for (int i = 0; i < 3; ++i) lam_in.f[vf[i]][a] += res[1 + i];
-    lam_in.f[4][a] += res[4];
-  }
-  // cell-averaged transverse fields
-  float *lbb[3] = {lbb0, lbb1, lbb2};
-  lbb[D1][a] += res[5];
-  lbb[D2][a] += res[6];
 }
 
-// Zero the cell-average accumulators (a fill kernel, capturable in graphs).
-__global__ void k_adj_zero3(long n, float *p0, float *p1, float *p2) {
-  const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= n) return;
-  p0[a] = 0.0f;
-  p1[a] = 0.0f;
-  p2[a] = 0.0f;
+// lam_in of rho, v, p from the three directions' reconstruction reverses
+// (written, or added to with accumulate), and the cell-average B adjoints
+// lbb[t] (written).
+__global__ void __launch_bounds__(256) k_adj_gather(Box b, FieldPtrs W, AdjFaceOut3 fa,
+                                                    FieldPtrs lam_in, int accumulate,
+                                                    float *lbb0, float *lbb1, float *lbb2) {
+  int x, y, z;
+  long a;
+  if (!cell_of(b, x, y, z, a)) return;
+  float rho = 0.0f, v[3] = {0.0f, 0.0f, 0.0f}, p = 0.0f, bb[3] = {0.0f, 0.0f, 0.0f};
+  float res[7];
+  adj_gather<0>(b, W, fa.d[0], x, y, z, res);  // X: (rho, vx, vy, vz, p, Bby, Bbz)
+  rho += res[0], v[0] += res[1], v[1] += res[2], v[2] += res[3], p += res[4];
+  bb[1] += res[5], bb[2] += res[6];
+  adj_gather<1>(b, W, fa.d[1], x, y, z, res);  // Y: (rho, vy, vz, vx, p, Bbz, Bbx)
+  rho += res[0], v[1] += res[1], v[2] += res[2], v[0] += res[3], p += res[4];
+  bb[2] += res[5], bb[0] += res[6];
+  adj_gather<2>(b, W, fa.d[2], x, y, z, res);  // Z: (rho, vz, vx, vy, p, Bbx, Bby)
+  rho += res[0], v[2] += res[1], v[0] += res[2], v[1] += res[3], p += res[4];
+  bb[0] += res[5], bb[1] += res[6];
+  const float out[5] = {rho, v[0], v[1], v[2], p};
+  for (int f = 0; f < 5; ++f) lam_in.f[f][a] = accumulate ? lam_in.f[f][a] + out[f] : out[f];
+  lbb0[a] = bb[0];
+  lbb1[a] = bb[1];
+  lbb2[a] = bb[2];
 }
 
 // Face B_t of lam_in: half of each adjacent cell's cell-average adjoint plus
@@ -391,32 +411,19 @@ void launch_adj_stage(hipStream_t s, const Box &b, const AdjStageArgs &A) {
                        A.gU[1], A.gU[2], A.gU[3], A.gU[4], A.gB[0], A.gB[1], A.gB[2], A.lam_base);
   hipLaunchKernelGGL(k_adj_edges, g, blk, 0, s, b, A.gB[0], A.gB[1], A.gB[2], A.h, A.lE[0],
                      A.lE[1], A.lE[2]);
-  hipLaunchKernelGGL(k_adj_zero3, g, blk, 0, s, b.ncell, A.lbb[0], A.lbb[1], A.lbb[2]);
+  AdjFaceOut3 fo;
   for (int d = 0; d < 3; ++d) {
-    AdjFaceOut fo;
     for (int k = 0; k < 7; ++k) {
-      fo.aL[k] = A.face[k];
-      fo.aR[k] = A.face[7 + k];
+      fo.d[d].aL[k] = A.face[d * 14 + k];
+      fo.d[d].aR[k] = A.face[d * 14 + 7 + k];
     }
-    fo.abn = A.abn[d];
-    const int first = (d == 0 && !A.acc_in) ? 1 : 0;
-    if (d == 0) {
-      hipLaunchKernelGGL(k_adj_face<0>, g, blk, 0, s, b, A.Win, A.gU[0], A.gU[1], A.gU[2],
-                         A.gU[3], A.gU[4], A.lE[0], A.lE[1], A.lE[2], A.h, fo);
-      hipLaunchKernelGGL(k_adj_gather<0>, g, blk, 0, s, b, A.Win, fo, A.lam_in, first, A.lbb[0],
-                         A.lbb[1], A.lbb[2]);
-    } else if (d == 1) {
-      hipLaunchKernelGGL(k_adj_face<1>, g, blk, 0, s, b, A.Win, A.gU[0], A.gU[1], A.gU[2],
-                         A.gU[3], A.gU[4], A.lE[0], A.lE[1], A.lE[2], A.h, fo);
-      hipLaunchKernelGGL(k_adj_gather<1>, g, blk, 0, s, b, A.Win, fo, A.lam_in, first, A.lbb[0],
-                         A.lbb[1], A.lbb[2]);
-    } else {
-      hipLaunchKernelGGL(k_adj_face<2>, g, blk, 0, s, b, A.Win, A.gU[0], A.gU[1], A.gU[2],
-                         A.gU[3], A.gU[4], A.lE[0], A.lE[1], A.lE[2], A.h, fo);
-      hipLaunchKernelGGL(k_adj_gather<2>, g, blk, 0, s, b, A.Win, fo, A.lam_in, first, A.lbb[0],
-                         A.lbb[1], A.lbb[2]);
-    }
+    fo.d[d].abn = A.abn[d];
   }
+  const dim3 g3(grid_for(b.ncell), 3);
+  hipLaunchKernelGGL(k_adj_faces, g3, blk, 0, s, b, A.Win, A.gU[0], A.gU[1], A.gU[2], A.gU[3],
+                     A.gU[4], A.lE[0], A.lE[1], A.lE[2], A.h, fo);
+  hipLaunchKernelGGL(k_adj_gather, g, blk, 0, s, b, A.Win, fo, A.lam_in, A.acc_in ? 1 : 0,
+                     A.lbb[0], A.lbb[1], A.lbb[2]);
   hipLaunchKernelGGL(k_adj_bfinish, g, blk, 0, s, b, A.lbb[0], A.lbb[1], A.lbb[2], A.abn[0],
                      A.abn[1], A.abn[2], A.lam_in, A.acc_in ? 1 : 0);
 }

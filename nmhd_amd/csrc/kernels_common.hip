@@ -193,12 +193,14 @@ __global__ void k_hlld_batch(long n, float gamma, FluxIn in, FluxOut out) {
 }
 
 // Copy the state (used when an enqueued step is inactive: the rotation of
-// the state buffers still has to carry the state forward).
+// the state buffers still has to carry the state forward).  A small grid
+// strides over the box, so that the usual case (the step was active and the
+// kernel returns at once) costs one short launch.
 __global__ void k_copy_state(Box b, FieldPtrs src, FieldPtrs dst, const StepState *st) {
   if (st->active) return;
-  const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (a >= b.ncell) return;
-  for (int f = 0; f < 8; ++f) dst.f[f][a] = src.f[f][a];
+  for (long a = (long)blockIdx.x * blockDim.x + threadIdx.x; a < b.ncell;
+       a += (long)gridDim.x * blockDim.x)
+    for (int f = 0; f < 8; ++f) dst.f[f][a] = src.f[f][a];
 }
 
 // Diagnostic entry points run on their own step record: current time of the
@@ -252,7 +254,8 @@ void launch_commit(hipStream_t s, StepState *st) {
 }
 void launch_copy_state(hipStream_t s, const Box &b, FieldPtrs src, FieldPtrs dst,
                        const StepState *st) {
-  hipLaunchKernelGGL(k_copy_state, dim3(grid_for(b.ncell)), dim3(kBlock), 0, s, b, src, dst,
+  const unsigned g = grid_for(b.ncell);
+  hipLaunchKernelGGL(k_copy_state, dim3(g < 2048 ? g : 2048), dim3(kBlock), 0, s, b, src, dst,
                      st);
 }
 void launch_hlld_batch(hipStream_t s, long n, float gamma, bool strict, FluxIn in,

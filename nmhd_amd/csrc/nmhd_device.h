@@ -167,14 +167,16 @@ __device__ __forceinline__ Flux7 hlld(float gamma, float inv_gm1, float bx,
   const float eLs = (sLu * eL - ptL * unL + pts * SM + bx * (vbL - vbLs)) * isLm;
   const float eRs = (sRu * eR - ptR * unR + pts * SM + bx * (vbR - vbRs)) * isRm;
 
-  // double-star (rotational) states
-  const float srL = fsqrt(rLs), srR = fsqrt(rRs);
-  const float SLs = SM - abx * frcp(srL);
-  const float SRs = SM + abx * frcp(srR);
+  // double-star (rotational) states; sqrt(rho*) and 1/sqrt(rho*) from one
+  // reciprocal square root each
+  const float isrL = __builtin_amdgcn_rsqf(rLs), isrR = __builtin_amdgcn_rsqf(rRs);
+  const float srL = rLs * isrL, srR = rRs * isrR;
+  const float SLs = SM - abx * isrL;
+  const float SRs = SM + abx * isrR;
   const float f = frcp(srL + srR);
   const float u1ss = f * (srL * u1Ls + srR * u1Rs + (b1Rs - b1Ls) * sgn);
   const float u2ss = f * (srL * u2Ls + srR * u2Rs + (b2Rs - b2Ls) * sgn);
-  const float srLR = fsqrt(rLs * rRs);
+  const float srLR = srL * srR;
   const float b1ss = f * (srL * b1Rs + srR * b1Ls + srLR * (u1Rs - u1Ls) * sgn);
   const float b2ss = f * (srL * b2Rs + srR * b2Ls + srLR * (u2Rs - u2Ls) * sgn);
   const float vbss = SM * bx + u1ss * b1ss + u2ss * b2ss;

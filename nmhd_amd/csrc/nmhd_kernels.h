@@ -180,7 +180,8 @@ struct AdjStageArgs {
   FieldPtrs lam_in;            // dL/dW_in through P (written, or added to with acc_in)
   float h;
   bool acc_base, acc_in;
-  float *gU[5], *gB[3], *lE[3], *lbb[3], *abn[3], *face[14];  // work arrays (ncell each)
+  // work arrays (ncell floats each); face: 14 per direction (aL[7], aR[7])
+  float *gU[5], *gB[3], *lE[3], *lbb[3], *abn[3], *face[42];
 };
 void launch_adj_set_dt(hipStream_t s, StepState *st, float dt);
 void launch_adj_stage(hipStream_t s, const Box &b, const AdjStageArgs &A);

@@ -251,6 +251,10 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
   const int c = (ty + 2) * LX + tx + 2;
   // z-stencil queue: q[s] holds plane p-3+s: rho, vz, vx, vy, p, Bx_avg, By_avg
   float q[4][7];
+  // limited z-slope of plane p-2 (carried from the previous plane: it is the
+  // left state's slope of this plane's z-face and the right state's of the
+  // previous one)
+  float zs_prev[7] = {};
   float bz_prev = 0.0f;  // B_z on the lower z-face of plane p-1
   float bz_cur = 0.0f;
   // divergence pipeline: x+y part of planes p-1, p-2; z-flux of the lower
@@ -319,10 +323,22 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
     // z-face below plane p-1 (one plane beyond the chunk: the upper face of
     // its last plane)
     float fzn[5] = {};
+    // limited slope of plane p-1, minmod(x_{p-1} - x_{p-2}, x_p - x_{p-1})
+    float zs_cur[7];
+#pragma unroll
+    for (int v = 0; v < 7; ++v) zs_cur[v] = minmod(q[2][v] - q[1][v], q[3][v] - q[2][v]);
     if (valid && p - 1 >= z0 && p - 1 <= zend) {
       float L[7], Rr[7];
 #pragma unroll
-      for (int v = 0; v < 7; ++v) recon(mode, q[0][v], q[1][v], q[2][v], q[3][v], L[v], Rr[v]);
+      for (int v = 0; v < 7; ++v) {
+        if (mode == 1) {
+          L[v] = q[1][v];
+          Rr[v] = q[2][v];
+        } else {
+          L[v] = q[1][v] + 0.5f * zs_prev[v];
+          Rr[v] = q[2][v] - 0.5f * zs_cur[v];
+        }
+      }
       // the z acceleration is zero: the face predictor adds nothing
       const Flux7 r = hlld(b.gamma, b.inv_gm1, bz_prev, L[0], Rr[0], L[1], Rr[1], L[2], Rr[2],
                            L[3], Rr[3], L[4], Rr[4], L[5], Rr[5], L[6], Rr[6]);
@@ -363,6 +379,8 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
       pxy1[k] = pxy[k];
       fzl[k] = fzn[k];
     }
+#pragma unroll
+    for (int v = 0; v < 7; ++v) zs_prev[v] = zs_cur[v];
     if (prefetch) store_plane(cb);  // plane p is no longer needed
     __syncthreads();
   }

@@ -2,6 +2,8 @@
 MHDSolver drop-in (proceed / save_state / load_state / read / write) on the
 GPU through libnmhd.so.  The programs are built by __graft_entry__.build()."""
 import os
+
+import numpy as np
 import subprocess
 
 import pytest
@@ -60,3 +62,10 @@ def test_kepler_disk_app_continue(gpu_ready, tmp_path):
     assert len(mon) >= 5
     by2 = [float(l.split("By_squared_average:")[1]) for l in mon]
     assert all(v >= 0 for v in by2) and max(by2) < 1e-3
+    # the dump post-processor reads the checkpoint back: one radial rank, so
+    # the output grid holds the interior (ghost rows fall outside the box)
+    from nmhd_amd import postprocess as PP
+    grid, arrays = PP.accumulate(os.path.join(str(tmp_path), "c"))
+    sx, sy, sz, fields = PP.read_bin(os.path.join(str(tmp_path), "c", "0000.bin"))
+    assert grid.size == [16, 16, 16] and (sx, sy, sz) == (16, 24, 16)
+    assert np.allclose(arrays, fields[:, :, 4:20, :], rtol=0, atol=0)
