@@ -304,6 +304,19 @@ int nmhd_adj_forward(nmhd_adj *adj, const float *const w_in[8], float dt,
 /* lam_in = (dS/dW at w_in)^T lam_out; recomputes the stage states from w_in. */
 int nmhd_adj_reverse(nmhd_adj *adj, const float *const w_in[8], float dt,
                      const float *const lam_out[8], float *const lam_in[8]);
+/* No reference counterpart (the reference has no derivatives).  As
+ * nmhd_adj_forward, also storing the midpoint stage state W* into w_stage
+ * (unused, may be NULL, for NMHD_EULER1). */
+int nmhd_adj_forward_stages(nmhd_adj *adj, const float *const w_in[8], float dt,
+                            float *const w_stage[8], float *const w_out[8]);
+/* As nmhd_adj_reverse, with this step's stage states supplied instead of
+ * recomputed: w_stage and w_out as nmhd_adj_forward_stages wrote them for the
+ * same w_in and dt (w_stage unused, may be NULL, for NMHD_EULER1).  The
+ * result is bitwise that of nmhd_adj_reverse; the forward kernels are not
+ * run again. */
+int nmhd_adj_reverse_stages(nmhd_adj *adj, const float *const w_in[8],
+                            const float *const w_stage[8], const float *const w_out[8],
+                            float dt, const float *const lam_out[8], float *const lam_in[8]);
 
 #ifdef __cplusplus
 }

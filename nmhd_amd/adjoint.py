@@ -15,9 +15,12 @@ dt is frozen: the forward pass records each step's CFL dt (or takes the
 given ones), and the backward pass reuses them.  Checkpointing keeps a
 subset of the states of the forward pass and recomputes the others in the
 backward pass:
-  "all"      every state (memory N, no recomputation);
+  "all"      every state and every step's midpoint stage state (memory
+             2N, no recomputation);
   "uniform"  every `every`-th state (default round(sqrt(N))): memory
-             N/every + every, one recomputed forward pass;
+             N/every + 2 every, one recomputed forward pass that keeps the
+             segment's stage states, so the reverse steps run no forward
+             kernels;
   "revolve"  binomial checkpointing with `snapshots` slots (default
              ceil(log2 N)): the split points minimise the number of
              recomputed steps for that memory (the offline optimum that
@@ -33,6 +36,7 @@ import torch
 from . import capi
 
 _FPA = ctypes.c_void_p * 8
+_OPA = ctypes.POINTER(ctypes.c_void_p)  # an optional array of 8 pointers (None = NULL)
 
 
 def _sig(L):
@@ -49,6 +53,8 @@ def _sig(L):
         "nmhd_adj_cfl_dt": [V, _FPA, ctypes.c_float, FP],
         "nmhd_adj_forward": [V, _FPA, ctypes.c_float, _FPA],
         "nmhd_adj_reverse": [V, _FPA, ctypes.c_float, _FPA, _FPA],
+        "nmhd_adj_forward_stages": [V, _FPA, ctypes.c_float, _OPA, _FPA],
+        "nmhd_adj_reverse_stages": [V, _FPA, _OPA, _FPA, ctypes.c_float, _FPA, _FPA],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -134,6 +140,36 @@ class Stepper:
                                             _ptrs(lam_in)), self.h)
         return lam_in
 
+    def two_stage(self):
+        return self._desc.integrator != capi.EULER1
+
+    def forward_stages(self, W, dt):
+        """(W*, S(W; dt)): the step's midpoint stage state (None for the
+        one-stage integrator) and its result."""
+        self._shape_ok(W)
+        self._on_torch_stream()
+        out = torch.empty_like(W)
+        ws = torch.empty_like(W) if self.two_stage() else None
+        self._check(_lib().nmhd_adj_forward_stages(self.h, _ptrs(W), float(dt),
+                                                   _ptrs(ws) if ws is not None else None,
+                                                   _ptrs(out)), self.h)
+        return ws, out
+
+    def reverse_stages(self, W, Ws, Wout, dt, lam_out):
+        """reverse() with the step's stage states (from forward_stages) given
+        instead of recomputed; bitwise the same result."""
+        for X in (W, Wout, lam_out):
+            self._shape_ok(X)
+        if self.two_stage():
+            self._shape_ok(Ws)
+        self._on_torch_stream()
+        lam_out = lam_out.contiguous()
+        lam_in = torch.empty_like(W)
+        self._check(_lib().nmhd_adj_reverse_stages(
+            self.h, _ptrs(W), _ptrs(Ws) if Ws is not None else None, _ptrs(Wout), float(dt),
+            _ptrs(lam_out), _ptrs(lam_in)), self.h)
+        return lam_in
+
 
 # ------------------------------------------------------------ checkpointing
 
@@ -189,13 +225,17 @@ class _Rollout(torch.autograd.Function):
     def forward(ctx, W0, stepper, dts, checkpoint, every, snapshots):
         n = len(dts)
         keep = {0: W0.detach()}
+        stage = {}
         W = W0.detach()
         for k in range(n):
-            W = stepper.forward(W, dts[k])
+            if checkpoint == "all":
+                stage[k], W = stepper.forward_stages(W, dts[k])
+            else:
+                W = stepper.forward(W, dts[k])
             if checkpoint == "all" or (checkpoint == "uniform" and (k + 1) % every == 0
                                        and k + 1 < n):
                 keep[k + 1] = W
-        ctx.stepper, ctx.dts, ctx.keep = stepper, list(dts), keep
+        ctx.stepper, ctx.dts, ctx.keep, ctx.stage = stepper, list(dts), keep, stage
         ctx.checkpoint, ctx.every, ctx.snapshots = checkpoint, every, snapshots
         return W
 
@@ -206,18 +246,27 @@ class _Rollout(torch.autograd.Function):
         lam = gN.contiguous()
         if ctx.checkpoint == "revolve":
             lam = reverse_segment(stepper, keep[0], 0, n, dts, lam, ctx.snapshots)
+        elif ctx.checkpoint == "all":
+            for j in range(n - 1, -1, -1):
+                lam = stepper.reverse_stages(keep[j], ctx.stage[j], keep[j + 1], dts[j], lam)
         else:
+            # one forward pass over the segment keeps every step's stage
+            # states, so no step's reverse runs the forward kernels again
             starts = sorted(keep)
             for i in range(len(starts) - 1, -1, -1):
                 a = starts[i]
                 b = starts[i + 1] if i + 1 < len(starts) else n
-                states = [keep[a]]
-                for j in range(a, b - 1):
-                    states.append(stepper.forward(states[-1], dts[j]))
+                states, stages = [keep[a]], []
+                for j in range(a, b):
+                    ws, w = stepper.forward_stages(states[-1], dts[j])
+                    stages.append(ws)
+                    states.append(w)
                 for j in range(b - 1, a - 1, -1):
-                    lam = stepper.reverse(states[j - a], dts[j], lam)
-                del states
-        ctx.keep = None
+                    lam = stepper.reverse_stages(states[j - a], stages[j - a],
+                                                 states[j - a + 1], dts[j], lam)
+                    states[j - a + 1] = stages[j - a] = None
+                del states, stages
+        ctx.keep = ctx.stage = None
         return lam, None, None, None, None, None
 
 

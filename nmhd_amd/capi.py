@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libnmhd.so")
 NMHD_OK = 0
 ERRORS = {1: "EINVAL", 2: "EHIP", 3: "ENOMEM", 4: "EIO", 5: "ESTATE"}
 PATH_AUTO, PATH_UNFUSED, PATH_TILED, PATH_STRICT = 0, 1, 2, 3
+RK2, EULER1 = 0, 1
 
 # Every entry point declared in include/nmhd.h.
 EXPORTS = (
@@ -31,7 +32,7 @@ EXPORTS = (
     "nmhd_batch_autotune", "nmhd_batch_run",
     "nmhd_adj_create", "nmhd_adj_destroy", "nmhd_adj_last_error", "nmhd_adj_get_geometry",
     "nmhd_adj_set_stream", "nmhd_adj_synchronize", "nmhd_adj_cfl_dt", "nmhd_adj_forward",
-    "nmhd_adj_reverse",
+    "nmhd_adj_reverse", "nmhd_adj_forward_stages", "nmhd_adj_reverse_stages",
 )
 TILE_64x4, TILE_32x8, TILE_16x16 = 0, 1, 2
 

@@ -1,8 +1,10 @@
 // capi_adjoint.hip -- the nmhd_adj_* entry points of include/nmhd.h: the
 // forward step with a frozen dt and its hand-written reverse, on caller-owned
 // device state arrays (e.g. torch tensors), launched on a caller-chosen
-// stream.  The reverse step recomputes the stage states it needs (W*, W^{n+1})
-// from W^n with the forward kernels, then runs the stage reverses
+// stream.  The reverse step needs the step's stage states (W*, W^{n+1}):
+// nmhd_adj_reverse recomputes them from W^n with the forward kernels,
+// nmhd_adj_reverse_stages takes them from the caller (as
+// nmhd_adj_forward_stages stored them); both then run the stage reverses
 // (kernels_adjoint.hip) from the last stage to the first.
 #include <hip/hip_runtime.h>
 
@@ -215,6 +217,54 @@ int nmhd_adj_forward(nmhd_adj *a, const float *const w_in[8], float dt, float *c
   return NMHD_OK;
 }
 
+int nmhd_adj_forward_stages(nmhd_adj *a, const float *const w_in[8], float dt,
+                            float *const w_stage[8], float *const w_out[8]) {
+  if (!a || !ptrs_ok(w_in) || !w_out || !ptrs_ok(w_out)) return afail(a, NMHD_EINVAL, "bad argument");
+  const bool two = a->desc.integrator != NMHD_EULER1;
+  if (two && (!w_stage || !ptrs_ok(w_stage))) return afail(a, NMHD_EINVAL, "bad argument");
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 8; ++j)
+      if (w_in[i] == w_out[j] || (two && (w_stage[i] == w_in[j] || w_stage[i] == w_out[j])))
+        return afail(a, NMHD_EINVAL, "w_in, w_stage and w_out must not alias");
+  enqueue_forward(a, to_fields_c(w_in), two ? to_fields(w_stage) : to_fields(a->wstar),
+                  to_fields(w_out), dt);
+  AHIP(a, hipGetLastError());
+  return NMHD_OK;
+}
+
+namespace {
+
+// lam_in = (dS/dW at Wn)^T lam_out given the step's stage states Ws, Wnext.
+void enqueue_reverse(nmhd_adj *a, FieldPtrs Wn, FieldPtrs Ws, FieldPtrs Wnext, float dt,
+                     FieldPtrs lam_out, FieldPtrs lam_in) {
+  AdjStageArgs A = stage_args(a);
+  if (a->desc.integrator == NMHD_EULER1) {
+    A.Win = Wn, A.Wbase = Wn, A.Wout = Wnext;
+    A.lam_out = lam_out;
+    A.lam_base = lam_in;
+    A.lam_in = lam_in;
+    A.h = dt, A.acc_base = false, A.acc_in = true;
+    launch_adj_stage(a->stream, a->box, A);
+    return;
+  }
+  // stage 2: W^{n+1} = prim(cons(W^n) + dt P(W*))
+  A.Win = Ws, A.Wbase = Wn, A.Wout = Wnext;
+  A.lam_out = lam_out;
+  A.lam_base = lam_in;
+  A.lam_in = to_fields(a->lstar);
+  A.h = dt, A.acc_base = false, A.acc_in = false;
+  launch_adj_stage(a->stream, a->box, A);
+  // stage 1: W* = prim(cons(W^n) + dt/2 P(W^n))
+  A.Win = Wn, A.Wbase = Wn, A.Wout = Ws;
+  A.lam_out = to_fields(a->lstar);
+  A.lam_base = lam_in;
+  A.lam_in = lam_in;
+  A.h = dt / 2, A.acc_base = true, A.acc_in = true;
+  launch_adj_stage(a->stream, a->box, A);
+}
+
+}  // namespace
+
 int nmhd_adj_reverse(nmhd_adj *a, const float *const w_in[8], float dt,
                      const float *const lam_out[8], float *const lam_in[8]) {
   if (!a || !ptrs_ok(w_in) || !ptrs_ok(lam_out) || !lam_in || !ptrs_ok(lam_in))
@@ -227,30 +277,25 @@ int nmhd_adj_reverse(nmhd_adj *a, const float *const w_in[8], float dt,
   const FieldPtrs Ws = to_fields(a->wstar);
   const FieldPtrs Wnext = to_fields(a->wnext);
   enqueue_forward(a, Wn, Ws, Wnext, dt);
-  AdjStageArgs A = stage_args(a);
-  if (a->desc.integrator == NMHD_EULER1) {
-    A.Win = Wn, A.Wbase = Wn, A.Wout = Wnext;
-    A.lam_out = to_fields_c(lam_out);
-    A.lam_base = to_fields(lam_in);
-    A.lam_in = to_fields(lam_in);
-    A.h = dt, A.acc_base = false, A.acc_in = true;
-    launch_adj_stage(a->stream, a->box, A);
-  } else {
-    // stage 2: W^{n+1} = prim(cons(W^n) + dt P(W*))
-    A.Win = Ws, A.Wbase = Wn, A.Wout = Wnext;
-    A.lam_out = to_fields_c(lam_out);
-    A.lam_base = to_fields(lam_in);
-    A.lam_in = to_fields(a->lstar);
-    A.h = dt, A.acc_base = false, A.acc_in = false;
-    launch_adj_stage(a->stream, a->box, A);
-    // stage 1: W* = prim(cons(W^n) + dt/2 P(W^n))
-    A.Win = Wn, A.Wbase = Wn, A.Wout = Ws;
-    A.lam_out = to_fields(a->lstar);
-    A.lam_base = to_fields(lam_in);
-    A.lam_in = to_fields(lam_in);
-    A.h = dt / 2, A.acc_base = true, A.acc_in = true;
-    launch_adj_stage(a->stream, a->box, A);
-  }
+  enqueue_reverse(a, Wn, Ws, Wnext, dt, to_fields_c(lam_out), to_fields(lam_in));
+  AHIP(a, hipGetLastError());
+  return NMHD_OK;
+}
+
+int nmhd_adj_reverse_stages(nmhd_adj *a, const float *const w_in[8],
+                            const float *const w_stage[8], const float *const w_out[8],
+                            float dt, const float *const lam_out[8], float *const lam_in[8]) {
+  const bool two = a && a->desc.integrator != NMHD_EULER1;
+  if (!a || !ptrs_ok(w_in) || !ptrs_ok(w_out) || (two && !ptrs_ok(w_stage)) ||
+      !ptrs_ok(lam_out) || !lam_in || !ptrs_ok(lam_in))
+    return afail(a, NMHD_EINVAL, "bad argument");
+  for (int i = 0; i < 8; ++i)
+    for (int j = 0; j < 8; ++j)
+      if (lam_in[i] == lam_out[j] || lam_in[i] == w_in[j] || lam_in[i] == w_out[j] ||
+          (two && lam_in[i] == w_stage[j]))
+        return afail(a, NMHD_EINVAL, "lam_in must not alias a state or lam_out");
+  enqueue_reverse(a, to_fields_c(w_in), two ? to_fields_c(w_stage) : to_fields(a->wstar),
+                  to_fields_c(w_out), dt, to_fields_c(lam_out), to_fields(lam_in));
   AHIP(a, hipGetLastError());
   return NMHD_OK;
 }

@@ -130,9 +130,12 @@ __device__ __forceinline__ Flux7 hlld(float gamma, float inv_gm1, float bx,
   // fast magnetosonic speeds and outer signal speeds
   const float gpL = gamma * pL, gpR = gamma * pR;
   const float gpbL = gpL + B2L, gpbR = gpR + B2R;
-  const float cfL = fsqrt(fdiv(gpbL + fsqrt(gpbL * gpbL - 4.0f * gpL * bx2), 2.0f * rL));
-  const float cfR = fsqrt(fdiv(gpbR + fsqrt(gpbR * gpbR - 4.0f * gpR * bx2), 2.0f * rR));
-  const float cmax = 1.41421356f * fmaxf(cfL, cfR);
+  // cf^2 = a / (2 rho); sqrt(2) max(cfL, cfR) = sqrt(a / rho) of the side
+  // with the larger a / rho (rho > 0): one reciprocal and one root
+  const float aL = gpbL + fsqrt(gpbL * gpbL - 4.0f * gpL * bx2);
+  const float aR = gpbR + fsqrt(gpbR * gpbR - 4.0f * gpR * bx2);
+  const bool lmax = aL * rR >= aR * rL;
+  const float cmax = fsqrt(fdiv(lmax ? aL : aR, lmax ? rL : rR));
   const float SL = fminf(unL, unR) - cmax;
   const float SR = fmaxf(unL, unR) + cmax;
 

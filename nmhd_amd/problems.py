@@ -194,6 +194,31 @@ def config_c2(n=(256, 256, 256), cfl=0.4):
 BATCH_EDGES = (16, 24, 32, 48, 64, 96)
 
 
+def shock_tube(axis=0, n=512, recon=0, integrator=0):
+    """The reference's MHD shock-tube demo (application/mhd_shock_demo:
+    simulation_config.hrb, mhd_solver.curb:9-27): n cells along `axis` (the
+    normal, extent 2, corner -1), one cell across, periodic, gamma 5/3, CFL
+    0.5; rho, p = 1, 1 for x < 0 and 0.125, 0.1 beyond; B_normal = 0.75,
+    B along the next axis +1 / -1, along the previous axis 0; v = 0 (the
+    Brio-Wu states).  The demo runs to t = 0.1.  Returns (cfg, W)."""
+    size = [1, 1, 1]
+    size[axis] = n
+    extent = [1.0, 1.0, 1.0]
+    extent[axis] = 2.0
+    cfg = make_config(size, global_extent=extent, cfl=0.5, recon=recon, integrator=integrator)
+    W = empty_state(cfg)
+    x = -1.0 + (np.arange(n, dtype=np.float64) + 0.5) * (2.0 / n)  # cell centres
+    left = x < 0
+    shape = [1, 1, 1]
+    shape[2 - axis] = n  # arrays are (z, y, x)
+    t1 = (axis + 1) % 3
+    W[0] = np.where(left, 1.0, 0.125).reshape(shape)
+    W[4] = np.where(left, 1.0, 0.1).reshape(shape)
+    W[5 + axis] = 0.75
+    W[5 + t1] = np.where(left, 1.0, -1.0).reshape(shape)
+    return cfg, W
+
+
 def batch_shapes(seed=1004, nbox=64, edges=BATCH_EDGES):
     """Edge lengths (nx, ny, nz) of each box, i.i.d. uniform over `edges`
     per axis from the counter RNG (component = axis, idx = box)."""

@@ -51,3 +51,42 @@ def test_batch_create_rejects_non_periodic_without_gpu():
         capi.Batch([good, bad])
     with pytest.raises(capi.NmhdError, match="integrator"):
         capi.Batch([good, P.make_config((8, 8, 8), integrator=1)])
+
+
+def _bad_desc_cases():
+    from nmhd_amd import problems as P
+    base = P.make_config((8, 8, 8))
+    cases = []
+
+    def case(name, edit, msg):
+        cases.append((name, edit, msg))
+
+    case("zero size", lambda d: d.global_size.__setitem__(1, 0), b"global_size")
+    case("negative margin", lambda d: d.margin.__setitem__(2, -1), b"margin")
+    case("zero extent", lambda d: d.global_extent.__setitem__(0, 0.0), b"global_extent")
+    case("gamma 1", lambda d: setattr(d, "gamma", 1.0), b"gamma")
+    case("cfl 0", lambda d: setattr(d, "cfl_factor", 0.0), b"cfl_factor")
+    case("source 7", lambda d: setattr(d, "source", 7), b"source")
+    case("recon 5", lambda d: setattr(d, "recon", 5), b"recon")
+    case("integrator 3", lambda d: setattr(d, "integrator", 3), b"integrator")
+    case("path 9", lambda d: setattr(d, "path", 9), b"path")
+    case("shear without margin", lambda d: setattr(d, "shear_y", 1), b"margin[1]")
+    case("margin beyond interior", lambda d: d.margin.__setitem__(1, 9), b"margin[1]")
+    case("2^31 cells", lambda d: [d.global_size.__setitem__(i, 1291) for i in range(3)],
+         b"2^31")
+    return base, cases
+
+
+def test_create_rejects_every_invalid_descriptor_without_gpu():
+    """Each field of the descriptor is range-checked on the host before any
+    device call, with a message naming the field."""
+    import ctypes
+    base, cases = _bad_desc_cases()
+    L = capi.load()
+    for name, edit, msg in cases:
+        d = capi.desc_from_config(base)
+        edit(d)
+        h = ctypes.c_void_p()
+        rc = L.nmhd_create(ctypes.byref(d), ctypes.byref(h))
+        assert rc == 1 and not h.value, name
+        assert msg in L.nmhd_last_error(None), (name, L.nmhd_last_error(None))

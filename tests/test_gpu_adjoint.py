@@ -11,7 +11,8 @@ binding nmhd_amd.adjoint):
   from the cancellation in p = (gamma-1)(E - rho v^2/2 - B^2/2));
 * the gradient of the magnetic energy after several steps (through
   torch.autograd with checkpointing) against the same FP64 reference;
-* every checkpointing scheme gives the same gradient bit for bit;
+* the stage-keeping forward/reverse pair equals the recomputing one bit for
+  bit, and every checkpointing scheme gives the same gradient bit for bit;
 * central finite differences (eps = 1e-3) along smooth random directions
   agree with the adjoint's directional derivatives within 1e-3, on data
   where the discrete step is differentiable along them: a bulk flow keeps
@@ -111,6 +112,33 @@ def test_rollout_gradient_vs_fp64_autograd(gpu_ready):
     T.magnetic_loss(X).backward()
     errs = group_norm_err(W0.grad.cpu(), W64.grad)
     assert errs.max() < 1e-3, errs
+
+
+@pytest.mark.parametrize("integrator", [0, 1])
+def test_stage_keeping_forward_and_reverse(gpu_ready, integrator):
+    announce("forward_stages/reverse_stages == forward/reverse bitwise, integrator %d"
+             % integrator)
+    cfg = P.make_config((16, 12, 8), cfl=0.4, integrator=integrator)
+    W = P.adjoint_case(cfg)
+    S = A.Stepper(cfg)
+    Wt = cuda(W)
+    dt = S.cfl_dt(Wt)
+    ws, wn = S.forward_stages(Wt, dt)
+    assert (ws is None) == (integrator == 1)
+    ref = S.forward(Wt, dt)
+    lam_out = cuda(P.smooth_direction(cfg, 79, 0, fields=tuple(range(8))))
+    got = S.reverse_stages(Wt, ws, wn, dt, lam_out)
+    want = S.reverse(Wt, dt, lam_out)
+    torch.cuda.synchronize()
+    bits = lambda X: X.cpu().numpy().view(np.uint32)
+    assert np.array_equal(bits(wn), bits(ref))
+    assert np.array_equal(bits(got), bits(want))
+    if ws is not None:  # rejected on the host, before any launch
+        L = A._lib()
+        assert L.nmhd_adj_reverse_stages(S.h, A._ptrs(Wt), A._ptrs(ws), A._ptrs(wn), dt,
+                                         A._ptrs(lam_out), A._ptrs(wn)) == 1  # lam_in = w_out
+        assert L.nmhd_adj_reverse_stages(S.h, A._ptrs(Wt), None, A._ptrs(wn), dt,
+                                         A._ptrs(lam_out), A._ptrs(got)) == 1  # no w_stage
 
 
 def test_checkpoint_schemes_bit_identical(gpu_ready):
