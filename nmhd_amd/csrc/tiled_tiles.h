@@ -109,6 +109,7 @@ __host__ __device__ inline StageBuf stage_buf(const FluxSet &F) {
 template <int TX, int TY>
 struct FacesShape {
   static_assert(TX * TY == kTileThreads, "tiles have 256 threads");
+  static_assert(TX <= 64 && TY <= 64, "a boundary face row or column fits one wave");
   static constexpr int LX = TX + 4, LY = TY + 4, LN = LX * LY;
   static constexpr int NS = (LN + kTileThreads - 1) / kTileThreads;  // slots per thread
   // LDS (floats): two staged planes of the 8 fields, then the hydro fluxes
@@ -310,14 +311,18 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
           SB.fb[3][a] = r.bt2;  // B_x
         }
       }
-      // the tile's upper boundary faces
-      if (tid < TY) {
-        const int yy = y0 + tid;
-        if (yy < ny && x0 + TX <= nx) put_x(tid, TX, face_x(cb, nb, (tid + 2) * LX + TX + 2));
-      } else if (tid < TY + TX) {
-        const int col = tid - TY, yy = y0 + TY;
-        if (x0 + col < nx && yy <= ny)
-          put_y(TY, col, face_y(cb, nb, (TY + 2) * LX + col + 2, yy < ny ? yy : yy - ny));
+      // the tile's upper boundary faces: the x-faces of column x0+TX on the
+      // first TY lanes of one wave, the y-faces of row y0+TY on the first TX
+      // lanes of another; the two waves rotate with the plane so that the
+      // SIMDs of a CU share the extra HLLD evaluations evenly
+      const int lane = tid & 63, wave = tid >> 6;
+      if (wave == ((2 * p) & 3) && lane < TY) {
+        const int yy = y0 + lane;
+        if (yy < ny && x0 + TX <= nx) put_x(lane, TX, face_x(cb, nb, (lane + 2) * LX + TX + 2));
+      } else if (wave == ((2 * p + 1) & 3) && lane < TX) {
+        const int yy = y0 + TY;
+        if (x0 + lane < nx && yy <= ny)
+          put_y(TY, lane, face_y(cb, nb, (TY + 2) * LX + lane + 2, yy < ny ? yy : yy - ny));
       }
     }
     // z-face below plane p-1 (one plane beyond the chunk: the upper face of
@@ -481,15 +486,24 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
 #pragma unroll
         for (int k = 0; k < 4; ++k) SP[k * RN + tid + s * NT] = rp[s][k];
   };
-  // own-column registers: hydro divergence and base cell values of plane r
+  // own-column registers: hydro divergence and base cell values of plane r.
+  // Values the rings already hold are read from LDS instead of memory: the
+  // base B_x, B_y (SP) always, the base velocities (S3) when the stage input
+  // is the base state (the first stage); the base B_z of plane r is the
+  // upper B_z of the previous plane.
   const int own = valid ? y * nx + x : 0;
-  float pc[5], pc_nx[5], wb[8], wb_nx[8], bz_hi = 0.0f, bz_nx = 0.0f;
+  const bool in_is_base = Win.f[1] == Wb.f[1] && Win.f[2] == Wb.f[2] && Win.f[3] == Wb.f[3];
+  float pc[5], pc_nx[5], wb[5] = {}, wb_nx[5] = {}, bz_lo = 0.0f, bz_hi = 0.0f, bz_nx = 0.0f;
   auto load_own = [&](int p, float *dpc, float *dwb) {
     const int base = wrap(p, nz) * plane + own;
 #pragma unroll
     for (int k = 0; k < 5; ++k) dpc[k] = SB.pace[k][base];
+    dwb[0] = Wb.f[0][base];
+    dwb[4] = Wb.f[4][base];
+    if (!in_is_base) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) dwb[k] = Wb.f[k][base];
+      for (int k = 1; k < 4; ++k) dwb[k] = Wb.f[k][base];
+    }
   };
 
   // prologue: planes z0-1, z0, z0+1 of the 3-ring, faces z0, z0+1, plane z0
@@ -506,6 +520,7 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
   loadp(z0);
   storep();
   load_own(z0, pc, wb);
+  bz_lo = Wb.f[7][wrap(z0, nz) * plane + own];
   bz_hi = Wb.f[7][wrap(z0 + 1, nz) * plane + own];
   __syncthreads();
 
@@ -584,10 +599,13 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
         Wo.f[6][a] = pby[0];
         Wo.f[7][a] = pbz[0];
       } else {
-        const float rho0 = wb[0], vx0 = wb[1], vy0 = wb[2], vz0 = wb[3];
-        const float bxl0 = wb[5], bxh0 = BXB[c + 1];
-        const float byl0 = wb[6], byh0 = BYB[c + RX];
-        const float bzl0 = wb[7], bzh0 = bz_hi;
+        const float rho0 = wb[0];
+        const float vx0 = in_is_base ? VX[1][c] : wb[1];
+        const float vy0 = in_is_base ? VY[1][c] : wb[2];
+        const float vz0 = in_is_base ? VZ[1][c] : wb[3];
+        const float bxl0 = BXB[c], bxh0 = BXB[c + 1];
+        const float byl0 = BYB[c], byh0 = BYB[c + RX];
+        const float bzl0 = bz_lo, bzh0 = bz_hi;
         const float e0 = wb[4] * b.inv_gm1 + 0.5f * rho0 * (vx0 * vx0 + vy0 * vy0 + vz0 * vz0) +
                          0.25f * (bxl0 * bxl0 + bxh0 * bxh0 + byl0 * byl0 + byh0 * byh0 +
                                   bzl0 * bzl0 + bzh0 * bzh0);
@@ -649,7 +667,8 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
 #pragma unroll
       for (int k = 0; k < 5; ++k) pc[k] = pc_nx[k];
 #pragma unroll
-      for (int k = 0; k < 8; ++k) wb[k] = wb_nx[k];
+      for (int k = 0; k < 5; ++k) wb[k] = wb_nx[k];
+      bz_lo = bz_hi;
       bz_hi = bz_nx;
     }
     __syncthreads();
