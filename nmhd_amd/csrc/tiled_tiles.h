@@ -117,7 +117,10 @@ struct FacesShape {
   static constexpr int kPlanes = 2 * 8 * LN;
   static constexpr int kFx = kPlanes;
   static constexpr int kFy = kFx + 5 * TY * (TX + 1);
-  static constexpr int kLds = kFy + 5 * (TY + 1) * TX;
+  // then the global in-plane offset of every staged slot (int), kept in LDS
+  // rather than in registers (read once per plane by the slot's own thread)
+  static constexpr int kGxy = kFy + 5 * (TY + 1) * TX;
+  static constexpr int kLds = kGxy + LN;
 };
 
 // Face fluxes of a TX x TY column tile over planes [z0, zend), and the
@@ -146,13 +149,14 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
   const bool src = b.source == 1;
   const float idx = b.inv_d[0], idy = b.inv_d[1], idz = b.inv_d[2];
 
-  // global in-plane offset of every LDS slot this thread stages
-  int gxy[NS];
+  // global in-plane offset of every LDS slot this thread stages (each thread
+  // writes and reads only its own slots)
+  int *const GXY = reinterpret_cast<int *>(lds + S::kGxy);
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const int li = tid + s * NT;
     const int lx = li % LX, ly = li / LX;
-    gxy[s] = (li < LN) ? pmod(y0 - 2 + ly, ny) * nx + pmod(x0 - 2 + lx, nx) : 0;
+    if (li < LN) GXY[li] = pmod(y0 - 2 + ly, ny) * nx + pmod(x0 - 2 + lx, nx);
   }
   float stage[NS][8];
   auto load_plane = [&](int p) {
@@ -160,8 +164,9 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       if (tid + s * NT < LN) {
+        const int g = base + GXY[tid + s * NT];
 #pragma unroll
-        for (int f = 0; f < 8; ++f) stage[s][f] = W.f[f][base + gxy[s]];
+        for (int f = 0; f < 8; ++f) stage[s][f] = W.f[f][g];
       }
     }
   };
