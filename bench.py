@@ -39,6 +39,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+# non-packed FP32 VALU issue peak: 256 CUs x 4 SIMDs x 16 lanes x 2.4 GHz
+VALU_PEAK = 256 * 4 * 16 * 2.4e9
 STEP_BYTES = 160.0  # algorithmic bytes per zone-update (8 fields x 5 passes x 4 B)
 # Algorithmic bytes per interior cell of one launch of each kernel, per path.
 KERNEL_BYTES = {
@@ -115,19 +117,19 @@ def cpu_baseline(cfg, W, steps):
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_shearing_box_256.json")
 
 
-def measured_traffic(kernel, size, path):
-    """HBM bytes per launch of `kernel` from the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE summary of this workload (None when the summary
-    does not cover this size, path and kernel)."""
+def measured_profile(kernel, size, path):
+    """The committed rocprofv3 record of `kernel` for this workload: HBM
+    bytes per launch (FETCH_SIZE / WRITE_SIZE, scaled) and VALU wave
+    instructions per launch (SQ_INSTS_VALU); {} when the summary does not
+    cover this size, path and kernel."""
     try:
         with open(TRAFFIC_FILE) as f:
             t = json.load(f)
     except (OSError, ValueError):
-        return None
+        return {}
     if list(t.get("global_size", [])) != list(size) or t.get("path") != path:
-        return None
-    k = t.get("kernels", {}).get(kernel)
-    return None if k is None else k["bytes_per_launch"]
+        return {}
+    return t.get("kernels", {}).get(kernel) or {}
 
 
 def batched_leg(a, torch):
@@ -357,11 +359,20 @@ def main():
     dom = max(kern, key=lambda k: kern[k][0])
     mean_s = kern[dom][0] / kern[dom][1] / 1e3
     achieved = kbytes[dom] * zones / mean_s
+    prof = measured_profile(dom, cfg["global_size"], path_name)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9,
                 "unit": "GB/s", "frac": achieved / HBM_PEAK,
-                "traffic": measured_traffic(dom, cfg["global_size"], path_name),
+                "traffic": prof.get("bytes_per_launch"),
                 "kernel": dom, "mean_launch_ms": mean_s * 1e3,
                 "algorithmic_bytes_per_launch": kbytes[dom] * zones}
+    if "valu_insts" in prof:
+        # the bound that binds this kernel: its VALU issue rate (instruction
+        # count per launch from the profile, time live) against the
+        # non-packed FP32 peak
+        lane_ops = prof["valu_insts"] * 64.0
+        roofline["valu"] = {"lane_ops_per_launch": lane_ops,
+                            "achieved": lane_ops / mean_s / 1e12, "peak": VALU_PEAK / 1e12,
+                            "unit": "Tlane-op/s", "frac": lane_ops / mean_s / VALU_PEAK}
     step_achieved = value * STEP_BYTES
     out = {
         "metric": "zone-updates/s at 256^3 fp32, % HBM roofline; batched & adjoint vs plain",

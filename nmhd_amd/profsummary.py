@@ -42,12 +42,14 @@ SLOTS = ((("k_faces<", ""), ("faces_s1", "faces_s2")),
          (("k_update_march<", ", false, false>"), ("update_s1",)))
 
 
-def traffic(fetch_db, write_db, fetch_scale, write_scale):
+def traffic(fetch_db, write_db, fetch_scale, write_scale, valu_db=None):
     """HBM bytes per launch per step slot: FETCH_SIZE * fetch_scale +
     WRITE_SIZE * write_scale (KiB per dispatch -> bytes).  The scales are
-    what the counters read for a copy kernel of known size."""
+    what the counters read for a copy kernel of known size.  With valu_db
+    (a run with SQ_INSTS_VALU) also the VALU wave instructions per launch."""
     f = counters(fetch_db)
     w = counters(write_db)
+    v = counters(valu_db) if valu_db else {}
     out = {}
     for (pat, suffix), slots in SLOTS:
         for k in f:
@@ -57,6 +59,8 @@ def traffic(fetch_db, write_db, fetch_scale, write_scale):
                 rec = {"rocprof_kernel": k, "fetch_kib": fk, "write_kib": wk,
                        "fetch_scale": fetch_scale, "write_scale": write_scale,
                        "bytes_per_launch": 1024.0 * (fk * fetch_scale + wk * write_scale)}
+                if "SQ_INSTS_VALU" in v.get(k, {}):
+                    rec["valu_insts"] = v[k]["SQ_INSTS_VALU"]
                 for sl in slots:
                     out[sl] = rec
     return out
@@ -71,6 +75,7 @@ def main():
     ap.add_argument("--write")
     ap.add_argument("--fetch-scale", type=float, default=1.0)
     ap.add_argument("--write-scale", type=float, default=1.0)
+    ap.add_argument("--valu", help="a --pmc run with SQ_INSTS_VALU (traffic-json mode)")
     ap.add_argument("--global-size", type=int, nargs=3, default=(256, 256, 256))
     ap.add_argument("--path", default="tiled")
     ap.add_argument("--command", default="")
@@ -78,7 +83,7 @@ def main():
     if a.traffic_json:
         import json
         rec = {"global_size": list(a.global_size), "path": a.path, "command": a.command,
-               "kernels": traffic(a.fetch, a.write, a.fetch_scale, a.write_scale)}
+               "kernels": traffic(a.fetch, a.write, a.fetch_scale, a.write_scale, a.valu)}
         with open(a.traffic_json, "w") as fh:
             json.dump(rec, fh, indent=1)
         return
