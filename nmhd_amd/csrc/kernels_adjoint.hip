@@ -240,7 +240,7 @@ __device__ __forceinline__ void adj_face(const Box &b, const FieldPtrs &W, const
       Lq[k] = q[k][1];
       Rq[k] = q[k][2];
     } else {
-      recon_minbee2(q[k][0], q[k][1], q[k][2], q[k][3], Lq[k], Rq[k]);
+      recon_minbee2_med3(q[k][0], q[k][1], q[k][2], q[k][3], Lq[k], Rq[k]);
     }
   }
   adj::HIn<float> in;

@@ -98,6 +98,21 @@ __device__ __forceinline__ void recon_minbee2(float x0, float x1, float x2, floa
   right = x2 - 0.5f * minmod(d12, x3 - x2);
 }
 
+// The same limiter as one median-of-three instruction: med3(a, b, 0) is 0
+// when a and b differ in sign and otherwise the one nearer 0, i.e. minmod.
+// It differs from minmod() only where a*b underflows to 0 (minmod: 0) and
+// in the sign of a zero result; the fast paths use it.
+__device__ __forceinline__ float minmod_med3(float a, float b) {
+  return __builtin_amdgcn_fmed3f(a, b, 0.0f);
+}
+
+__device__ __forceinline__ void recon_minbee2_med3(float x0, float x1, float x2, float x3,
+                                                   float &left, float &right) {
+  const float d12 = x2 - x1;
+  left = x1 + 0.5f * minmod_med3(x1 - x0, d12);
+  right = x2 - 0.5f * minmod_med3(d12, x3 - x2);
+}
+
 // HLLD flux across a face with normal field bx (Miyoshi & Kusano 2005) with
 // the outer signal speeds min/max(vL, vR) -/+ sqrt(2) max(cfL, cfR), the
 // 1e-10 guard in the single-star denominators and signBx = 0 for bx = 0.

@@ -55,7 +55,7 @@ __device__ __forceinline__ void recon(int mode, float x0, float x1, float x2, fl
     l = x1;
     r = x2;
   } else {
-    recon_minbee2(x0, x1, x2, x3, l, r);
+    recon_minbee2_med3(x0, x1, x2, x3, l, r);
   }
 }
 
@@ -336,7 +336,7 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
     // limited slope of plane p-1, minmod(x_{p-1} - x_{p-2}, x_p - x_{p-1})
     float zs_cur[7];
 #pragma unroll
-    for (int v = 0; v < 7; ++v) zs_cur[v] = minmod(q[2][v] - q[1][v], q[3][v] - q[2][v]);
+    for (int v = 0; v < 7; ++v) zs_cur[v] = minmod_med3(q[2][v] - q[1][v], q[3][v] - q[2][v]);
     if (valid && p - 1 >= z0 && p - 1 <= zend) {
       float L[7], Rr[7];
 #pragma unroll

@@ -86,7 +86,10 @@ __global__ void __launch_bounds__(256) k_flux(Box b, FieldPtrs W, FluxOut F, con
       L[v] = q[v][1];
       R[v] = q[v][2];
     } else {
-      recon_minbee2(q[v][0], q[v][1], q[v][2], q[v][3], L[v], R[v]);
+      if constexpr (Strict)
+        recon_minbee2(q[v][0], q[v][1], q[v][2], q[v][3], L[v], R[v]);
+      else
+        recon_minbee2_med3(q[v][0], q[v][1], q[v][2], q[v][3], L[v], R[v]);
     }
   }
   if (b.source == 1 && D != 2) {
