@@ -187,7 +187,7 @@ void launch_adj_set_dt(hipStream_t s, StepState *st, float dt);
 void launch_adj_stage(hipStream_t s, const Box &b, const AdjStageArgs &A);
 
 // kernels_tiled.hip
-int tiled_zchunk(const Box &b);
+int tiled_zchunk(const Box &b, int tx, int ty, bool faces);
 // tile: TileVariant of the z-marching tiles
 void launch_faces(hipStream_t s, const Box &b, FieldPtrs W, const FluxSet &F, const StepState *st,
                   int hsel, int tile = TV_16x16);
