@@ -49,6 +49,11 @@ __device__ __forceinline__ int pmod(int i, int n) {
   return r < 0 ? r + n : r;
 }
 
+// Results that the next kernel reads are stored non-temporally: written
+// once per step, they would otherwise displace the halo planes that
+// neighbouring tiles re-read from L2 (1-3% faster per kernel, measured).
+__device__ __forceinline__ void put_nt(float *dst, float v) { __builtin_nontemporal_store(v, dst); }
+
 __device__ __forceinline__ void recon(int mode, float x0, float x1, float x2, float x3, float &l,
                                       float &r) {
   if (mode == 1) {
@@ -303,8 +308,8 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
         put_x(ty, tx, r);
         if (valid) {
           const int a = wrap(p, nz) * plane + y * nx + x;
-          SB.fb[0][a] = r.bt1;  // B_y
-          SB.fb[1][a] = r.bt2;  // B_z
+          put_nt(&SB.fb[0][a], r.bt1);  // B_y
+          put_nt(&SB.fb[1][a], r.bt2);  // B_z
         }
       }
       if (x < nx && y <= ny) {
@@ -312,8 +317,8 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
         put_y(ty, tx, r);
         if (valid) {
           const int a = wrap(p, nz) * plane + y * nx + x;
-          SB.fb[2][a] = r.bt1;  // B_z
-          SB.fb[3][a] = r.bt2;  // B_x
+          put_nt(&SB.fb[2][a], r.bt1);  // B_z
+          put_nt(&SB.fb[3][a], r.bt2);  // B_x
         }
       }
       // the tile's upper boundary faces: the x-faces of column x0+TX on the
@@ -359,8 +364,8 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
       fzn[4] = r.e;
       if (p - 1 < zend) {
         const int a = wrap(p - 1, nz) * plane + y * nx + x;
-        SB.fb[4][a] = r.bt1;  // B_x
-        SB.fb[5][a] = r.bt2;  // B_y
+        put_nt(&SB.fb[4][a], r.bt1);  // B_x
+        put_nt(&SB.fb[5][a], r.bt2);  // B_y
       }
     }
     __syncthreads();
@@ -381,7 +386,7 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
     if (valid && p - 2 >= z0 && p - 2 < zend) {
       const int a = wrap(p - 2, nz) * plane + y * nx + x;
 #pragma unroll
-      for (int k = 0; k < 5; ++k) SB.pace[k][a] = pxy2[k] + (fzl[k] - fzn[k]) * idz;
+      for (int k = 0; k < 5; ++k) put_nt(&SB.pace[k][a], pxy2[k] + (fzl[k] - fzn[k]) * idz);
     }
 #pragma unroll
     for (int k = 0; k < 5; ++k) {
@@ -599,10 +604,10 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
       }
       if (PaceOnly) {
 #pragma unroll
-        for (int k = 0; k < 5; ++k) Wo.f[k][a] = pc[k];
-        Wo.f[5][a] = pbx[0];
-        Wo.f[6][a] = pby[0];
-        Wo.f[7][a] = pbz[0];
+        for (int k = 0; k < 5; ++k) put_nt(&Wo.f[k][a], pc[k]);
+        put_nt(&Wo.f[5][a], pbx[0]);
+        put_nt(&Wo.f[6][a], pby[0]);
+        put_nt(&Wo.f[7][a], pbz[0]);
       } else {
         const float rho0 = wb[0];
         const float vx0 = in_is_base ? VX[1][c] : wb[1];
@@ -635,14 +640,14 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
           ux += h * ax_;
           uy += h * ay_;
         }
-        Wo.f[0][a] = rho;
-        Wo.f[1][a] = ux;
-        Wo.f[2][a] = uy;
-        Wo.f[3][a] = uz;
-        Wo.f[4][a] = pres;
-        Wo.f[5][a] = bxl;
-        Wo.f[6][a] = byl;
-        Wo.f[7][a] = bzl;
+        put_nt(&Wo.f[0][a], rho);
+        put_nt(&Wo.f[1][a], ux);
+        put_nt(&Wo.f[2][a], uy);
+        put_nt(&Wo.f[3][a], uz);
+        put_nt(&Wo.f[4][a], pres);
+        put_nt(&Wo.f[5][a], bxl);
+        put_nt(&Wo.f[6][a], byl);
+        put_nt(&Wo.f[7][a], bzl);
         if (Cfl) {
           // CFL time of the new state for cells the next ghost fill leaves
           // untouched (interior rows/planes); the ghost cells are reduced
