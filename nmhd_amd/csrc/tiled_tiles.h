@@ -53,6 +53,8 @@ __device__ __forceinline__ int pmod(int i, int n) {
 // once per step, they would otherwise displace the halo planes that
 // neighbouring tiles re-read from L2 (1-3% faster per kernel, measured).
 __device__ __forceinline__ void put_nt(float *dst, float v) { __builtin_nontemporal_store(v, dst); }
+// Likewise for values that only the loading thread reads (no halo reuse).
+__device__ __forceinline__ float get_nt(const float *src) { return __builtin_nontemporal_load(src); }
 
 __device__ __forceinline__ void recon(int mode, float x0, float x1, float x2, float x3, float &l,
                                       float &r) {
@@ -507,12 +509,12 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
   auto load_own = [&](int p, float *dpc, float *dwb) {
     const int base = wrap(p, nz) * plane + own;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) dpc[k] = SB.pace[k][base];
-    dwb[0] = Wb.f[0][base];
-    dwb[4] = Wb.f[4][base];
+    for (int k = 0; k < 5; ++k) dpc[k] = get_nt(&SB.pace[k][base]);
+    dwb[0] = get_nt(&Wb.f[0][base]);
+    dwb[4] = get_nt(&Wb.f[4][base]);
     if (!in_is_base) {
 #pragma unroll
-      for (int k = 1; k < 4; ++k) dwb[k] = Wb.f[k][base];
+      for (int k = 1; k < 4; ++k) dwb[k] = get_nt(&Wb.f[k][base]);
     }
   };
 
@@ -544,7 +546,7 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
       loadz(r + 2);
       loadp(r + 1);
       load_own(r + 1, pc_nx, wb_nx);
-      bz_nx = Wb.f[7][wrap(r + 2, nz) * plane + own];
+      bz_nx = get_nt(&Wb.f[7][wrap(r + 2, nz) * plane + own]);
     }
     const int sm = (r + 2) % 3, s0 = (r + 3) % 3, sp = (r + 4) % 3;  // planes r-1, r, r+1
     const int f0 = r & 1, f1 = (r + 1) & 1;                         // faces r, r+1
