@@ -208,7 +208,7 @@ int nmhd_timing_get(nmhd_ctx *ctx, int i, const char **name, double *total_ms,
 int nmhd_set_path(nmhd_ctx *ctx, int path);
 int nmhd_get_path(const nmhd_ctx *ctx, int *path);
 /* Tile shapes (enum nmhd_tile) of the face and update kernels of
- * NMHD_PATH_TILED (default NMHD_TILE_16x16 and NMHD_TILE_64x4); results do
+ * NMHD_PATH_TILED (default NMHD_TILE_32x8 and NMHD_TILE_64x4); results do
  * not depend on them. */
 int nmhd_set_tiles(nmhd_ctx *ctx, int faces_tile, int update_tile);
 int nmhd_get_tiles(const nmhd_ctx *ctx, int *faces_tile, int *update_tile);

@@ -127,8 +127,8 @@ struct nmhd_ctx {
   double *diag_part = nullptr;  // run diagnostics: per-block partials, result
   double *diag_out = nullptr;
   int path = NMHD_PATH_TILED;
-  // tile shapes of the tiled path (fastest measured at 256^3 on MI355X)
-  int ftile = TV_16x16, utile = TV_64x4;
+  // tile shapes of the tiled path (fastest measured at 256^3-512^3 on MI355X)
+  int ftile = TV_32x8, utile = TV_64x4;
   bool cfl_valid = false;  // st->cfl_next_bits holds the CFL minimum of the state
   std::string err;
   bool timing = false;

@@ -190,7 +190,7 @@ void launch_adj_stage(hipStream_t s, const Box &b, const AdjStageArgs &A);
 int tiled_zchunk(const Box &b, int tx, int ty, bool faces);
 // tile: TileVariant of the z-marching tiles
 void launch_faces(hipStream_t s, const Box &b, FieldPtrs W, const FluxSet &F, const StepState *st,
-                  int hsel, int tile = TV_16x16);
+                  int hsel, int tile = TV_32x8);
 void launch_update_tiled(hipStream_t s, const Box &b, FieldPtrs Win, const FluxSet &F,
                          FieldPtrs Wbase, FieldPtrs Wout, StepState *st, int hsel, bool pace_only,
                          bool cfl, int tile = TV_64x4);
