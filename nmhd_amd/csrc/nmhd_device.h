@@ -129,30 +129,35 @@ __device__ __forceinline__ Flux7 hlld(float gamma, float inv_gm1, float bx,
                                       float b2L, float b2R) {
   const float TINY = 1.0e-10f;
   const float abx = fabsf(bx);
-  const float sgn = bx > 0.0f ? 1.0f : (bx < 0.0f ? -1.0f : 0.0f);
+  // sign(bx), 0 for bx = 0 (a sign-bit copy and one select)
+  const float sgn = bx == 0.0f ? 0.0f : copysignf(1.0f, bx);
   const float bx2 = bx * bx;
 
-  // outer states
+  // outer states; E = p/(gamma-1) + (rho v^2 + B^2)/2, pt = p + B^2/2
   const float B2L = bx2 + b1L * b1L + b2L * b2L;
   const float B2R = bx2 + b1R * b1R + b2R * b2R;
   const float v2L = unL * unL + ut1L * ut1L + ut2L * ut2L;
   const float v2R = unR * unR + ut1R * ut1R + ut2R * ut2R;
-  const float eL = pL * inv_gm1 + 0.5f * rL * v2L + 0.5f * B2L;
-  const float eR = pR * inv_gm1 + 0.5f * rR * v2R + 0.5f * B2R;
+  const float kinL = rL * v2L + B2L, kinR = rR * v2R + B2R;
+  const float eL = pL * inv_gm1 + 0.5f * kinL;
+  const float eR = pR * inv_gm1 + 0.5f * kinR;
   const float ptL = pL + 0.5f * B2L;
   const float ptR = pR + 0.5f * B2R;
 
   // fast magnetosonic speeds and outer signal speeds
   const float gpL = gamma * pL, gpR = gamma * pR;
   const float gpbL = gpL + B2L, gpbR = gpR + B2R;
+  const float bx2x4 = 4.0f * bx2;
   // cf^2 = a / (2 rho); sqrt(2) max(cfL, cfR) = sqrt(a / rho) of the side
   // with the larger a / rho (rho > 0): one reciprocal and one root
-  const float aL = gpbL + fsqrt(gpbL * gpbL - 4.0f * gpL * bx2);
-  const float aR = gpbR + fsqrt(gpbR * gpbR - 4.0f * gpR * bx2);
+  const float aL = gpbL + fsqrt(gpbL * gpbL - gpL * bx2x4);
+  const float aR = gpbR + fsqrt(gpbR * gpbR - gpR * bx2x4);
   const bool lmax = aL * rR >= aR * rL;
   const float cmax = fsqrt(fdiv(lmax ? aL : aR, lmax ? rL : rR));
-  const float SL = fminf(unL, unR) - cmax;
-  const float SR = fmaxf(unL, unR) + cmax;
+  // min / max of the normal velocities by one compare (no NaN handling)
+  const bool lslow = unL < unR;
+  const float SL = (lslow ? unL : unR) - cmax;
+  const float SR = (lslow ? unR : unL) + cmax;
 
   // contact speed and star total pressure
   const float sLu = SL - unL, sRu = SR - unR;
