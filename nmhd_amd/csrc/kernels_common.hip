@@ -105,17 +105,8 @@ __device__ __forceinline__ float wave_min(float v) {
 // b2 = sum_d max(B_d,lo^2, B_d,hi^2), reduced to one float with a wave
 // shuffle min, an LDS block min and one atomic min per block on the float
 // bits (valid: all candidates are >= 0).
-__global__ void __launch_bounds__(256) k_cfl(Box b, FieldPtrs W, float goal, StepState *st) {
-  if (st->finished) return;
+__device__ __forceinline__ void cfl_block_fold(float t, StepState *st) {
   __shared__ float red[4];
-  const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  float t = goal;
-  if (a < b.ncell) {
-    const int x = (int)(a % b.n[0]);
-    const int y = (int)((a / b.n[0]) % b.n[1]);
-    const int z = (int)(a / b.n[0] / b.n[1]);
-    t = fminf(t, cell_cfl(b, W, a, x, y, z));
-  }
   t = wave_min(t);
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   if (lane == 0) red[wid] = t;
@@ -127,49 +118,49 @@ __global__ void __launch_bounds__(256) k_cfl(Box b, FieldPtrs W, float goal, Ste
   }
 }
 
+// Grid-stride over the cells with a capped grid: the per-block atomics on the
+// one result word serialise, so a grid of one block per 256 cells would
+// spend most of its time in them.
+__global__ void __launch_bounds__(256) k_cfl(Box b, FieldPtrs W, float goal, StepState *st) {
+  if (st->finished) return;
+  float t = goal;
+  for (long a = (long)blockIdx.x * blockDim.x + threadIdx.x; a < b.ncell;
+       a += (long)gridDim.x * blockDim.x) {
+    const int x = (int)(a % b.n[0]);
+    const int y = (int)((a / b.n[0]) % b.n[1]);
+    const int z = (int)(a / b.n[0] / b.n[1]);
+    t = fminf(t, cell_cfl(b, W, a, x, y, z));
+  }
+  cfl_block_fold(t, st);
+}
+
 // CFL over the cells a ghost fill rewrites: the Y ghost rows, and with Z
 // ghosts the Z ghost planes plus the last interior plane (whose upper B_z
 // face is a ghost face).  slab 0: nx * 2My * nz cells, slab 1:
 // nx * ny * (2Mz + 1) cells.
 __global__ void __launch_bounds__(256) k_cfl_ghosts(Box b, FieldPtrs W, int slab, StepState *st) {
   if (st->finished) return;
-  __shared__ float red[4];
-  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   const int nx = b.n[0], ny = b.n[1], nz = b.n[2];
+  const int M = slab == 0 ? b.m[1] : b.m[2];
+  const long n = slab == 0 ? (long)nx * 2 * M * nz : (long)nx * ny * (2 * M + 1);
   float t = 3.0e38f;
-  int x = 0, y = 0, z = 0;
-  bool ok = false;
-  if (slab == 0) {
-    const int M = b.m[1];
-    const long n = (long)nx * 2 * M * nz;
-    if (i < n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long)gridDim.x * blockDim.x) {
+    int x, y, z;
+    if (slab == 0) {
       x = (int)(i % nx);
       const int r = (int)((i / nx) % (2 * M));
       z = (int)(i / nx / (2 * M));
       y = r < M ? r : ny - 2 * M + r;
-      ok = true;
-    }
-  } else {
-    const int M = b.m[2];
-    const long n = (long)nx * ny * (2 * M + 1);
-    if (i < n) {
+    } else {
       x = (int)(i % nx);
       y = (int)((i / nx) % ny);
       const int r = (int)(i / nx / ny);
       z = r < M ? r : nz - 2 * M - 1 + r;
-      ok = true;
     }
+    t = fminf(t, cell_cfl(b, W, addr(b, x, y, z), x, y, z));
   }
-  if (ok) t = cell_cfl(b, W, addr(b, x, y, z), x, y, z);
-  t = wave_min(t);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) red[wid] = t;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    float m = red[0];
-    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) m = fminf(m, red[k]);
-    atomicMin(&st->cfl_bits, __float_as_uint(m));
-  }
+  cfl_block_fold(t, st);
 }
 
 __global__ void k_dt(StepState *st, float goal) { step_dt(*st, goal); }
@@ -225,14 +216,20 @@ __global__ void k_diag_prepare(StepState *diag, const StepState *st, float h) {
 void launch_begin_step(hipStream_t s, StepState *st, bool full) {
   hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, st, (int)full);
 }
+// Blocks of a reduction kernel: one per 1024 cells, at most 512.
+unsigned reduce_grid(long n) {
+  const unsigned g = grid_for(n, 4 * kBlock);
+  return g < 1 ? 1 : (g > 512 ? 512 : g);
+}
+
 void launch_cfl_ghosts(hipStream_t s, const Box &b, FieldPtrs W, StepState *st) {
   if (b.m[1] > 0) {
     const long n = (long)b.n[0] * 2 * b.m[1] * b.n[2];
-    hipLaunchKernelGGL(k_cfl_ghosts, dim3(grid_for(n)), dim3(kBlock), 0, s, b, W, 0, st);
+    hipLaunchKernelGGL(k_cfl_ghosts, dim3(reduce_grid(n)), dim3(kBlock), 0, s, b, W, 0, st);
   }
   if (b.m[2] > 0) {
     const long n = (long)b.n[0] * b.n[1] * (2 * b.m[2] + 1);
-    hipLaunchKernelGGL(k_cfl_ghosts, dim3(grid_for(n)), dim3(kBlock), 0, s, b, W, 1, st);
+    hipLaunchKernelGGL(k_cfl_ghosts, dim3(reduce_grid(n)), dim3(kBlock), 0, s, b, W, 1, st);
   }
 }
 void launch_ghost_y(hipStream_t s, const Box &b, FieldPtrs W, const StepState *st) {
@@ -244,7 +241,7 @@ void launch_ghost_z(hipStream_t s, const Box &b, FieldPtrs W, const StepState *s
   hipLaunchKernelGGL(k_ghost_z, dim3(grid_for(unit)), dim3(kBlock), 0, s, b, W, st);
 }
 void launch_cfl(hipStream_t s, const Box &b, FieldPtrs W, float goal, StepState *st) {
-  hipLaunchKernelGGL(k_cfl, dim3(grid_for(b.ncell)), dim3(kBlock), 0, s, b, W, goal, st);
+  hipLaunchKernelGGL(k_cfl, dim3(reduce_grid(b.ncell)), dim3(kBlock), 0, s, b, W, goal, st);
 }
 void launch_dt(hipStream_t s, StepState *st, float goal) {
   hipLaunchKernelGGL(k_dt, dim3(1), dim3(1), 0, s, st, goal);
