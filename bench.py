@@ -296,8 +296,9 @@ def stage_rates(cfg, tim, kbytes, zones):
 def main():
     a = parse()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
-              "(the multi-GPU decomposition is described in DESIGN.md)" % a.gpus,
+        print("bench.py: the step runs on one GPU; --gpus %d is not supported (a "
+              "multi-GPU run would shard the box in z-slabs with a 4-plane halo "
+              "exchange per stage and one dt all-reduce per step)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
     import torch

@@ -54,7 +54,7 @@ static std::vector<char> slurp(const std::string &f) {
 }
 
 int main(int argc, char **argv) {
-  const std::string dir = argc > 1 ? argv[1] : "gpurun_out/facade_test";
+  const std::string dir = argc > 1 ? argv[1] : "facade_test_out";
   const SimulationConfig cfg = ot_config(16);
   // 1. uninterrupted 6 steps
   {

@@ -10,7 +10,6 @@ import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 BIN = os.path.join(ROOT, "tests", "cpp", "bin")
-OUT = os.path.join(ROOT, "gpurun_out")
 
 
 def run(exe, *args, timeout=120):
