@@ -169,3 +169,18 @@ def test_batch_box_against_oracle(gpu_ready):
     bmax = np.abs(got[5:8]).max()
     assert np.abs(div_b(got, cfg["d"])).max() * max(cfg["d"]) < 1e-6 * bmax
 
+
+
+def test_batch_thin_boxes_equal_single_boxes(gpu_ready):
+    """Boxes one cell thick (the reference's shock-tube demo along x, y and
+    z, 128 cells long, one with first-order reconstruction) next to a 3D box:
+    tiles far wider than the box, periodic wrap onto the same cell."""
+    announce("batch of thin shock-tube boxes + a 3D box vs single contexts")
+    boxes = [P.shock_tube(0, 128), P.shock_tube(1, 128), P.shock_tube(2, 128, recon=1)]
+    cfg, W = P.shock_tube(1, 64)
+    boxes.append((cfg, W))
+    boxes.append(small_boxes()[2])
+    ref = single_runs(boxes, 0.05, 20)
+    B = make_batch(boxes)
+    B.run(0.05, 20, use_graph=True)
+    assert_batch_equals(B, ref)
