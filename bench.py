@@ -298,7 +298,7 @@ def main():
     if a.gpus != 1:
         print("bench.py: the step runs on one GPU; --gpus %d is not supported (a "
               "multi-GPU run would shard the box in z-slabs with a 4-plane halo "
-              "exchange per stage and one dt all-reduce per step)" % a.gpus,
+              "exchange and one dt all-reduce per step)" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
     import torch
