@@ -103,6 +103,13 @@ def cpu_baseline(cfg, W, steps):
     el = time.perf_counter() - t0
     orc.close()
     from nmhd_amd import problems as P
+    # the same step on one core, on a smaller box of the same setup
+    c1 = P.config_c2((128, 128, 128))
+    one = pyoracle.Oracle(c1, P.shearing_box_mri(c1, seed=1002), threads=1)
+    t1 = time.perf_counter()
+    one.step()
+    el1 = time.perf_counter() - t1
+    one.close()
     return {
         "value": P.interior_cells(cfg) * steps / el,
         "unit": "zone-updates/s",
@@ -111,6 +118,8 @@ def cpu_baseline(cfg, W, steps):
         "sample": "%d steps of the same %dx%dx%d shearing box on the oracle "
                   "(C restatement of the reference step, OpenMP, %d threads)"
                   % ((steps,) + tuple(cfg["global_size"]) + (threads,)),
+        "single_thread": {"value": P.interior_cells(c1) / el1, "unit": "zone-updates/s",
+                          "sample": "1 step of the 128^3 box of the same setup, 1 thread"},
     }
 
 
