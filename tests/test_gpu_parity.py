@@ -309,3 +309,31 @@ def test_enqueue_past_goal_keeps_state(gpu_ready):
     k2, _ = g.synchronize()
     assert reached and k2 == k
     assert np.array_equal(S1, g.get_state())
+
+
+def test_two_contexts_interleaved_equal_alone(gpu_ready):
+    """Contexts share nothing: two boxes stepped in alternation (each on its
+    own stream, enqueued without synchronisation in between) end bit for bit
+    where each ends alone."""
+    announce("two contexts interleaved vs alone")
+    cfgs = [P.config_c2((32, 32, 16)), P.make_config((24, 16, 20), cfl=0.4)]
+    states = [P.shearing_box_mri(cfgs[0], seed=21), P.orszag_tang_noisy(cfgs[1], seed=22)]
+    alone = []
+    for cfg, W in zip(cfgs, states):
+        g = capi.Context(cfg)
+        g.set_state(W)
+        g.run(1e30, 6)
+        alone.append(g.get_state())
+        g.close()
+    ctxs = [capi.Context(c) for c in cfgs]
+    for g, W in zip(ctxs, states):
+        g.set_state(W)
+        g.reset_run()
+    for _ in range(3):
+        for g in ctxs:
+            g.enqueue_steps(1e30, 2)
+    for g, ref in zip(ctxs, alone):
+        k, _ = g.synchronize()
+        assert k == 6
+        assert bitwise_equal(g.get_state(), ref)
+        g.close()
