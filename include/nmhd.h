@@ -246,7 +246,9 @@ int nmhd_batch_get_status(nmhd_batch *batch, int box, int *steps_taken, int *goa
 
 /* Tiling: tile shape (enum nmhd_tile) of box's face and update kernels
  * (box = -1: every box); z chunk (planes per work item) and workgroups per
- * CU of the two persistent launches.  Results do not depend on the tiling. */
+ * CU of the two persistent launches, where 0 (the default) launches one
+ * workgroup per work item instead, one launch per tile shape.  Results do
+ * not depend on the tiling or the launch mode. */
 int nmhd_batch_set_tiles(nmhd_batch *batch, int box, int faces_tile, int update_tile);
 int nmhd_batch_set_launch(nmhd_batch *batch, int faces_zchunk, int update_zchunk,
                           int faces_wg_per_cu, int update_wg_per_cu);

@@ -1,7 +1,9 @@
 // capi_batch.hip -- the nmhd_batch_* entry points of include/nmhd.h: many
 // independent periodic boxes in shared pools, per-box step records, a work
-// table of tiles for the two persistent stage kernels, HIP-graph replay of
-// the step, and the autotuner of tile shapes and launch parameters.
+// table of tiles for the stage kernels (grouped by tile shape: one launch per
+// shape with a workgroup per tile, or one persistent launch over the whole
+// table), HIP-graph replay of the step, and the autotuner of tile shapes and
+// launch parameters.
 //
 // Pools: state[k] (k = 0..2, the rotating W^n / W* / W^{n+1} copies) holds
 // the 8 fields of every box back to back (box b at BatchBox::off8, field f at
@@ -29,7 +31,7 @@ namespace {
 std::string g_batch_create_error;
 
 constexpr int kZChunks[3] = {8, 16, 32};
-constexpr int kWgPerCu[4] = {1, 2, 3, 4};
+constexpr int kWgPerCu[5] = {0, 1, 2, 3, 4};  // 0: one workgroup per item
 constexpr int kTuneItems = 48;  // work items timed per shape and tile shape
 
 }  // namespace
@@ -53,9 +55,12 @@ struct nmhd_batch {
   int cur = 0;
   int integrator = NMHD_MIDPOINT2;
   std::vector<int> ftile, utile;
-  int fzc = 16, uzc = 16, fwg = 2, uwg = 2;
+  // workgroups per CU of the persistent launches; 0: one workgroup per work
+  // item, one launch per tile shape
+  int fzc = 16, uzc = 16, fwg = 0, uwg = 0;
   WorkItem *d_fitems = nullptr, *d_uitems = nullptr, *d_titems = nullptr;
   int nf = 0, nu = 0;
+  int nfv[TV_COUNT] = {}, nuv[TV_COUNT] = {};  // items per tile shape (the lists are grouped)
   size_t cap_f = 0, cap_u = 0, cap_t = 0;
   bool items_dirty = true;
   bool cfl_valid = false;
@@ -81,23 +86,29 @@ int bfail(nmhd_batch *b, int code, const std::string &msg) {
   } while (0)
 
 // Work items of the given boxes: tiles of tile shape tile[box], z chunks of
-// zc planes, box by box, x fastest, then y, then z.
+// zc planes, grouped by tile shape, then box by box, x fastest, then y, then
+// z; counts[v] (optional) receives the number of items of shape v.
 std::vector<WorkItem> make_items(const nmhd_batch *b, const std::vector<int> &tile, int zc,
-                                 const std::vector<int> &which) {
+                                 const std::vector<int> &which, int *counts = nullptr) {
   std::vector<WorkItem> out;
-  for (int box : which) {
-    const Box &bx = b->boxes[box].b;
-    const int v = tile[box], TX = kTileTX[v], TY = kTileTY[v];
-    for (int z0 = 0; z0 < bx.n[2]; z0 += zc)
-      for (int y0 = 0; y0 < bx.n[1]; y0 += TY)
-        for (int x0 = 0; x0 < bx.n[0]; x0 += TX) {
-          WorkItem w;
-          w.box = box;
-          w.xy = x0 | (y0 << 16);
-          w.z = z0 | (std::min(z0 + zc, bx.n[2]) << 16);
-          w.variant = v;
-          out.push_back(w);
-        }
+  for (int v = 0; v < TV_COUNT; ++v) {
+    const size_t start = out.size();
+    const int TX = kTileTX[v], TY = kTileTY[v];
+    for (int box : which) {
+      if (tile[box] != v) continue;
+      const Box &bx = b->boxes[box].b;
+      for (int z0 = 0; z0 < bx.n[2]; z0 += zc)
+        for (int y0 = 0; y0 < bx.n[1]; y0 += TY)
+          for (int x0 = 0; x0 < bx.n[0]; x0 += TX) {
+            WorkItem w;
+            w.box = box;
+            w.xy = x0 | (y0 << 16);
+            w.z = z0 | (std::min(z0 + zc, bx.n[2]) << 16);
+            w.variant = v;
+            out.push_back(w);
+          }
+    }
+    if (counts) counts[v] = (int)(out.size() - start);
   }
   return out;
 }
@@ -132,8 +143,8 @@ void drop_graphs(nmhd_batch *b) {
 int refresh_items(nmhd_batch *b) {
   if (!b->items_dirty) return NMHD_OK;
   const std::vector<int> all = all_boxes(b);
-  const std::vector<WorkItem> f = make_items(b, b->ftile, b->fzc, all);
-  const std::vector<WorkItem> u = make_items(b, b->utile, b->uzc, all);
+  const std::vector<WorkItem> f = make_items(b, b->ftile, b->fzc, all, b->nfv);
+  const std::vector<WorkItem> u = make_items(b, b->utile, b->uzc, all, b->nuv);
   int r = upload_items(b, f, &b->d_fitems, &b->cap_f);
   if (r) return r;
   r = upload_items(b, u, &b->d_uitems, &b->cap_u);
@@ -149,6 +160,32 @@ int grid_of(int nitems, int wg_per_cu, int cus) {
   return std::max(1, std::min(nitems, wg_per_cu * cus));
 }
 
+// The face / update launches of one stage over a grouped item list: the
+// persistent kernel with wg workgroups per CU, or (wg == 0) one launch per
+// tile shape with one workgroup per item.
+void faces_over(nmhd_batch *b, hipStream_t s, int wg, const WorkItem *items, int n,
+                const int *counts, float *state, const StepState *st, int hsel) {
+  if (wg > 0) {
+    launch_batch_faces(s, grid_of(n, wg, b->cus), b->d_boxes, items, n, state, b->flux, st, hsel);
+    return;
+  }
+  for (int v = 0, off = 0; v < TV_COUNT; off += counts[v], ++v)
+    launch_batch_faces_items(s, v, b->d_boxes, items + off, counts[v], state, b->flux, st, hsel);
+}
+
+void update_over(nmhd_batch *b, hipStream_t s, int wg, const WorkItem *items, int n,
+                 const int *counts, float *in, float *base, float *out, StepState *st, int hsel,
+                 bool cfl) {
+  if (wg > 0) {
+    launch_batch_update(s, grid_of(n, wg, b->cus), b->d_boxes, items, n, in, b->flux, base, out,
+                        st, hsel, cfl);
+    return;
+  }
+  for (int v = 0, off = 0; v < TV_COUNT; off += counts[v], ++v)
+    launch_batch_update_items(s, v, b->d_boxes, items + off, counts[v], in, b->flux, base, out,
+                              st, hsel, cfl);
+}
+
 // One iteration of the proceed() loop for every box.
 void enqueue_step(nmhd_batch *b, bool full_cfl, float goal) {
   hipStream_t s = b->stream;
@@ -156,11 +193,10 @@ void enqueue_step(nmhd_batch *b, bool full_cfl, float goal) {
   launch_batch_begin(s, b->st, n, full_cfl);
   if (full_cfl) launch_batch_cfl(s, b->d_boxes, n, b->state[A], goal, b->st);
   launch_batch_dt(s, b->st, n, b->d_goal);
-  const int gf = grid_of(b->nf, b->fwg, b->cus), gu = grid_of(b->nu, b->uwg, b->cus);
   auto stage = [&](int in, int base, int out, int hsel, bool last) {
-    launch_batch_faces(s, gf, b->d_boxes, b->d_fitems, b->nf, b->state[in], b->flux, b->st, hsel);
-    launch_batch_update(s, gu, b->d_boxes, b->d_uitems, b->nu, b->state[in], b->flux,
-                        b->state[base], b->state[out], b->st, hsel, last);
+    faces_over(b, s, b->fwg, b->d_fitems, b->nf, b->nfv, b->state[in], b->st, hsel);
+    update_over(b, s, b->uwg, b->d_uitems, b->nu, b->nuv, b->state[in], b->state[base],
+                b->state[out], b->st, hsel, last);
   };
   int next;
   if (b->integrator == NMHD_EULER1) {
@@ -390,25 +426,22 @@ int tune(nmhd_batch *b, const char *path, int *timed) {
   if (hit == cache.entries.end() || hit->second.size() != 4) {
     const std::vector<int> all = all_boxes(b);
     float best_f = 1e30f, best_u = 1e30f;
-    int fzc = 16, uzc = 16, fwg = 2, uwg = 2;
+    int fzc = 16, uzc = 16, fwg = 0, uwg = 0;
     for (int zc : kZChunks) {
-      const std::vector<WorkItem> fi = make_items(b, b->ftile, zc, all);
+      int cf[TV_COUNT], cu[TV_COUNT];
+      const std::vector<WorkItem> fi = make_items(b, b->ftile, zc, all, cf);
       r = upload_items(b, fi, &b->d_titems, &b->cap_t);
       if (r) return r;
       for (int wg : kWgPerCu) {
         float ms = 0.0f;
         const int n = (int)fi.size();
         r = time_launch(
-            b,
-            [&] {
-              launch_batch_faces(b->stream, grid_of(n, wg, b->cus), b->d_boxes, b->d_titems, n,
-                                 in, b->flux, b->st_tune, H_HALF);
-            },
+            b, [&] { faces_over(b, b->stream, wg, b->d_titems, n, cf, in, b->st_tune, H_HALF); },
             &ms);
         if (r) return r;
         if (ms < best_f) best_f = ms, fzc = zc, fwg = wg;
       }
-      const std::vector<WorkItem> ui = make_items(b, b->utile, zc, all);
+      const std::vector<WorkItem> ui = make_items(b, b->utile, zc, all, cu);
       r = upload_items(b, ui, &b->d_titems, &b->cap_t);
       if (r) return r;
       for (int wg : kWgPerCu) {
@@ -417,8 +450,8 @@ int tune(nmhd_batch *b, const char *path, int *timed) {
         r = time_launch(
             b,
             [&] {
-              launch_batch_update(b->stream, grid_of(n, wg, b->cus), b->d_boxes, b->d_titems, n,
-                                  in, b->flux, in, scratch, b->st_tune, H_HALF, false);
+              update_over(b, b->stream, wg, b->d_titems, n, cu, in, in, scratch, b->st_tune,
+                          H_HALF, false);
             },
             &ms);
         if (r) return r;
@@ -615,7 +648,7 @@ int nmhd_batch_set_tiles(nmhd_batch *b, int box, int faces_tile, int update_tile
 }
 
 int nmhd_batch_set_launch(nmhd_batch *b, int fzc, int uzc, int fwg, int uwg) {
-  if (!b || fzc < 1 || uzc < 1 || fzc > 32767 || uzc > 32767 || fwg < 1 || uwg < 1 || fwg > 64 ||
+  if (!b || fzc < 1 || uzc < 1 || fzc > 32767 || uzc > 32767 || fwg < 0 || uwg < 0 || fwg > 64 ||
       uwg > 64)
     return bfail(b, NMHD_EINVAL, "bad launch parameters");
   b->fzc = fzc;

@@ -1,8 +1,11 @@
 // kernels_batch.hip -- many independent periodic boxes of uneven sizes
-// advanced together.  Each RK stage is two persistent launches (face fluxes,
-// then the update) whose workgroups walk a work table of tiles from every
-// box; a tile's box record gives its sizes, its per-box step (own dt) and the
-// offsets of its fields in the batch's pools.  The tile functions are the
+// advanced together.  Each RK stage runs the face fluxes, then the update,
+// over a work table of tiles from every box, either as one workgroup per
+// work item with one launch per tile shape (the kernel then needs only that
+// shape's registers and keeps the single-box occupancy), or as two
+// persistent launches whose workgroups walk the whole table; a tile's box
+// record gives its sizes, its per-box step (own dt) and the offsets of its
+// fields in the batch's pools.  The tile functions are the
 // single-box path's (tiled_tiles.h), so a box advanced in a batch is
 // bit-identical to the same box advanced alone.  The bookkeeping kernels run
 // one thread per box on the per-box step records.
@@ -98,6 +101,46 @@ __global__ void __launch_bounds__(tiled::kTileThreads)
   }
 }
 
+// One workgroup per work item of one tile shape.
+template <int TX, int TY>
+__global__ void __launch_bounds__(tiled::kTileThreads, 4)
+    k_batch_faces_items(const BatchBox *boxes, const WorkItem *items, int nitems, float *state,
+                        float *flux, const StepState *st, int hsel) {
+  __shared__ float lds[tiled::FacesShape<TX, TY>::kLds];
+  const int it = xcd_logical_block(blockIdx.x, gridDim.x);
+  if (it >= nitems) return;
+  const WorkItem w = items[it];
+  const StepState *s = st + w.box;
+  if (!s->active) return;
+  const BatchBox bb = boxes[w.box];
+  int x0, y0, z0, zend;
+  item_tile(w, x0, y0, z0, zend);
+  tiled::faces_tile<TX, TY>(bb.b, box_fields(state, bb), box_fluxes(flux, bb),
+                            tiled::stage_h(s, hsel), x0, y0, z0, zend, lds);
+}
+
+template <int TX, int TY, bool Cfl>
+__global__ void __launch_bounds__(tiled::kTileThreads)
+    k_batch_update_items(const BatchBox *boxes, const WorkItem *items, int nitems,
+                         float *state_in, float *flux, float *state_base, float *state_out,
+                         StepState *st, int hsel) {
+  __shared__ float lds[tiled::UpdateShape<TX, TY>::kLds];
+  __shared__ float red[4];
+  const int it = xcd_logical_block(blockIdx.x, gridDim.x);
+  if (it >= nitems) return;
+  const WorkItem w = items[it];
+  StepState *s = st + w.box;
+  if (!s->active) return;
+  const BatchBox bb = boxes[w.box];
+  int x0, y0, z0, zend;
+  item_tile(w, x0, y0, z0, zend);
+  float cfl_t = 3.0e38f;
+  tiled::update_tile<TX, TY, false, Cfl>(bb.b, box_fields(state_in, bb), box_fluxes(flux, bb),
+                                         box_fields(state_base, bb), box_fields(state_out, bb),
+                                         tiled::stage_h(s, hsel), x0, y0, z0, zend, lds, cfl_t);
+  if (Cfl) tiled::block_min_into(cfl_t, &s->cfl_next_bits, red);
+}
+
 __global__ void k_batch_begin(StepState *st, int nbox, int full) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < nbox) step_begin(st[i], full);
@@ -177,6 +220,51 @@ void launch_batch_faces(hipStream_t s, int grid, const BatchBox *boxes, const Wo
   hipLaunchKernelGGL(k_batch_faces, dim3(grid), dim3(tiled::kTileThreads), 0, s, boxes, items,
                      nitems, state, flux, st, hsel);
 }
+void launch_batch_faces_items(hipStream_t s, int variant, const BatchBox *boxes,
+                              const WorkItem *items, int nitems, float *state, float *flux,
+                              const StepState *st, int hsel) {
+  if (nitems <= 0) return;
+  const dim3 grid(nitems), blk(tiled::kTileThreads);
+  if (variant == TV_64x4)
+    hipLaunchKernelGGL((k_batch_faces_items<64, 4>), grid, blk, 0, s, boxes, items, nitems, state,
+                       flux, st, hsel);
+  else if (variant == TV_32x8)
+    hipLaunchKernelGGL((k_batch_faces_items<32, 8>), grid, blk, 0, s, boxes, items, nitems, state,
+                       flux, st, hsel);
+  else
+    hipLaunchKernelGGL((k_batch_faces_items<16, 16>), grid, blk, 0, s, boxes, items, nitems,
+                       state, flux, st, hsel);
+}
+
+template <bool Cfl>
+void update_items_as(hipStream_t s, int variant, const BatchBox *boxes, const WorkItem *items,
+                     int nitems, float *state_in, float *flux, float *state_base,
+                     float *state_out, StepState *st, int hsel) {
+  const dim3 grid(nitems), blk(tiled::kTileThreads);
+  if (variant == TV_64x4)
+    hipLaunchKernelGGL((k_batch_update_items<64, 4, Cfl>), grid, blk, 0, s, boxes, items, nitems,
+                       state_in, flux, state_base, state_out, st, hsel);
+  else if (variant == TV_32x8)
+    hipLaunchKernelGGL((k_batch_update_items<32, 8, Cfl>), grid, blk, 0, s, boxes, items, nitems,
+                       state_in, flux, state_base, state_out, st, hsel);
+  else
+    hipLaunchKernelGGL((k_batch_update_items<16, 16, Cfl>), grid, blk, 0, s, boxes, items,
+                       nitems, state_in, flux, state_base, state_out, st, hsel);
+}
+
+void launch_batch_update_items(hipStream_t s, int variant, const BatchBox *boxes,
+                               const WorkItem *items, int nitems, float *state_in, float *flux,
+                               float *state_base, float *state_out, StepState *st, int hsel,
+                               bool cfl) {
+  if (nitems <= 0) return;
+  if (cfl)
+    update_items_as<true>(s, variant, boxes, items, nitems, state_in, flux, state_base,
+                          state_out, st, hsel);
+  else
+    update_items_as<false>(s, variant, boxes, items, nitems, state_in, flux, state_base,
+                           state_out, st, hsel);
+}
+
 void launch_batch_update(hipStream_t s, int grid, const BatchBox *boxes, const WorkItem *items,
                          int nitems, float *state_in, float *flux, float *state_base,
                          float *state_out, StepState *st, int hsel, bool cfl) {

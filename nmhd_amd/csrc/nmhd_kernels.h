@@ -166,6 +166,14 @@ void launch_batch_faces(hipStream_t s, int grid, const BatchBox *boxes, const Wo
 void launch_batch_update(hipStream_t s, int grid, const BatchBox *boxes, const WorkItem *items,
                          int nitems, float *state_in, float *flux, float *state_base,
                          float *state_out, StepState *st, int hsel, bool cfl);
+// One workgroup per item, all items of one tile shape (variant).
+void launch_batch_faces_items(hipStream_t s, int variant, const BatchBox *boxes,
+                              const WorkItem *items, int nitems, float *state, float *flux,
+                              const StepState *st, int hsel);
+void launch_batch_update_items(hipStream_t s, int variant, const BatchBox *boxes,
+                               const WorkItem *items, int nitems, float *state_in, float *flux,
+                               float *state_base, float *state_out, StepState *st, int hsel,
+                               bool cfl);
 
 // kernels_diag.hip (run diagnostics: 12 FP64 values, see k_diag_partial)
 constexpr int kDiagVals = 12;

@@ -1,11 +1,11 @@
 """Batched boxes (nmhd_batch_*): many periodic boxes of uneven sizes in
-persistent launches over a work table, per-box dt, HIP-graph replay and the
-autotuner.
+per-tile-shape or persistent launches over a work table, per-box dt,
+HIP-graph replay and the autotuner.
 
 The batched kernels run the single-box tile functions, and a cell's
 arithmetic does not depend on the tiling, so the invariant checked here is
 bit-identity: a box advanced in a batch (eager or graph, any tile shape, z
-chunk or workgroup count, tuned or not) equals the same box advanced alone
+chunk, launch mode or workgroup count, tuned or not) equals the same box advanced alone
 by a Context on the tiled path, state and time, bit for bit.  The tiled path
 itself is checked against the oracle in test_gpu_parity.py; one batched box
 is compared with the oracle here as well.
@@ -100,6 +100,9 @@ def test_batch_graph_equals_eager(gpu_ready):
     (capi.TILE_64x4, capi.TILE_64x4, 8, 32, 1, 4),
     (capi.TILE_32x8, capi.TILE_16x16, 5, 7, 3, 1),
     (capi.TILE_16x16, capi.TILE_32x8, 32, 3, 2, 2),
+    (capi.TILE_32x8, capi.TILE_64x4, 8, 16, 0, 0),
+    (capi.TILE_64x4, capi.TILE_16x16, 3, 8, 0, 3),
+    (capi.TILE_16x16, capi.TILE_32x8, 16, 5, 2, 0),
 ])
 def test_batch_tilings_bit_identical(gpu_ready, ft, ut, fz, uz, fw, uw):
     announce("batch tiling faces %d update %d zc %d/%d wg %d/%d" % (ft, ut, fz, uz, fw, uw))
@@ -108,6 +111,19 @@ def test_batch_tilings_bit_identical(gpu_ready, ft, ut, fz, uz, fw, uw):
     B = make_batch(boxes)
     B.set_tiles(ft, ut)
     B.set_launch(fz, uz, fw, uw)
+    B.run(1e30, 3, use_graph=True)
+    assert_batch_equals(B, ref)
+
+
+@pytest.mark.parametrize("fw,uw", [(0, 0), (2, 2)])
+def test_batch_mixed_tile_shapes_bit_identical(gpu_ready, fw, uw):
+    announce("batch with a different tile shape per box, wg %d/%d" % (fw, uw))
+    boxes = small_boxes(seed=9)
+    ref = single_runs(boxes, 1e30, 3)
+    B = make_batch(boxes)
+    for i in range(B.nbox):
+        B.set_tiles(i % 3, (i + 1) % 3, box=i)
+    B.set_launch(8, 16, fw, uw)
     B.run(1e30, 3, use_graph=True)
     assert_batch_equals(B, ref)
 
