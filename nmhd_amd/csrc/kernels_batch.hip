@@ -51,13 +51,13 @@ __global__ void __launch_bounds__(tiled::kTileThreads)
     item_tile(w, x0, y0, z0, zend);
     switch (w.variant) {
       case TV_64x4:
-        tiled::faces_tile<64, 4>(bb.b, W, F, h, x0, y0, z0, zend, lds);
+        tiled::faces_tile_any<64, 4>(bb.b, W, F, h, x0, y0, z0, zend, lds);
         break;
       case TV_32x8:
-        tiled::faces_tile<32, 8>(bb.b, W, F, h, x0, y0, z0, zend, lds);
+        tiled::faces_tile_any<32, 8>(bb.b, W, F, h, x0, y0, z0, zend, lds);
         break;
       default:
-        tiled::faces_tile<16, 16>(bb.b, W, F, h, x0, y0, z0, zend, lds);
+        tiled::faces_tile_any<16, 16>(bb.b, W, F, h, x0, y0, z0, zend, lds);
         break;
     }
   }
@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(tiled::kTileThreads, 4)
   const BatchBox bb = boxes[w.box];
   int x0, y0, z0, zend;
   item_tile(w, x0, y0, z0, zend);
-  tiled::faces_tile<TX, TY>(bb.b, box_fields(state, bb), box_fluxes(flux, bb),
+  tiled::faces_tile_any<TX, TY>(bb.b, box_fields(state, bb), box_fluxes(flux, bb),
                             tiled::stage_h(s, hsel), x0, y0, z0, zend, lds);
 }
 

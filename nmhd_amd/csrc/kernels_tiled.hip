@@ -14,7 +14,8 @@
 namespace nmhd {
 namespace tiled {
 
-template <int TX, int TY>
+// Recon = b.recon (the launcher picks the instantiation)
+template <int TX, int TY, int Recon>
 __global__ void __launch_bounds__(kTileThreads, 4) k_faces(Box b, FieldPtrs W, FluxSet F,
                                                         const StepState *st, int hsel, int zc,
                                                         int gx, int gy) {
@@ -22,7 +23,7 @@ __global__ void __launch_bounds__(kTileThreads, 4) k_faces(Box b, FieldPtrs W, F
   __shared__ float lds[FacesShape<TX, TY>::kLds];
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int x0 = (lb % gx) * TX, y0 = ((lb / gx) % gy) * TY, z0 = (lb / gx / gy) * zc;
-  faces_tile<TX, TY>(b, W, F, stage_h(st, hsel), x0, y0, z0, min(z0 + zc, b.n[2]), lds);
+  faces_tile<TX, TY, Recon>(b, W, F, stage_h(st, hsel), x0, y0, z0, min(z0 + zc, b.n[2]), lds);
 }
 
 template <int TX, int TY, bool PaceOnly, bool Cfl>
@@ -66,8 +67,12 @@ void faces_as(hipStream_t s, const Box &b, FieldPtrs W, const FluxSet &F, const 
               int hsel) {
   const int zc = tiled_zchunk(b, TX, TY, true);
   const int gx = (b.n[0] + TX - 1) / TX, gy = (b.n[1] + TY - 1) / TY, gz = (b.n[2] + zc - 1) / zc;
-  hipLaunchKernelGGL((tiled::k_faces<TX, TY>), dim3(gx * gy * gz), dim3(tiled::kTileThreads), 0,
-                     s, b, W, F, st, hsel, zc, gx, gy);
+  if (b.recon == 1)
+    hipLaunchKernelGGL((tiled::k_faces<TX, TY, 1>), dim3(gx * gy * gz), dim3(tiled::kTileThreads),
+                       0, s, b, W, F, st, hsel, zc, gx, gy);
+  else
+    hipLaunchKernelGGL((tiled::k_faces<TX, TY, 0>), dim3(gx * gy * gz), dim3(tiled::kTileThreads),
+                       0, s, b, W, F, st, hsel, zc, gx, gy);
 }
 
 template <int TX, int TY>

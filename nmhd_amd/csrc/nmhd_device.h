@@ -133,14 +133,10 @@ __device__ __forceinline__ Flux7 hlld(float gamma, float inv_gm1, float bx,
   const float sgn = bx == 0.0f ? 0.0f : copysignf(1.0f, bx);
   const float bx2 = bx * bx;
 
-  // outer states; E = p/(gamma-1) + (rho v^2 + B^2)/2, pt = p + B^2/2
+  // outer states: pt = p + B^2/2 (the energy E = p/(gamma-1) + (rho v^2 +
+  // B^2)/2 is formed below, on the side the fan region takes)
   const float B2L = bx2 + b1L * b1L + b2L * b2L;
   const float B2R = bx2 + b1R * b1R + b2R * b2R;
-  const float v2L = unL * unL + ut1L * ut1L + ut2L * ut2L;
-  const float v2R = unR * unR + ut1R * ut1R + ut2R * ut2R;
-  const float kinL = rL * v2L + B2L, kinR = rR * v2R + B2R;
-  const float eL = pL * inv_gm1 + 0.5f * kinL;
-  const float eR = pR * inv_gm1 + 0.5f * kinR;
   const float ptL = pL + 0.5f * B2L;
   const float ptR = pR + 0.5f * B2R;
 
@@ -166,7 +162,7 @@ __device__ __forceinline__ Flux7 hlld(float gamma, float inv_gm1, float bx,
   const float SM = (sRu * rR * unR - sLu * rL * unL - ptR + ptL) * iden;
   const float pts = (sRu * rR * ptL - sLu * rL * ptR + rL * rR * sRu * sLu * (unR - unL)) * iden;
 
-  // single-star states
+  // single-star states (both sides: the double-star states need them)
   const float sLm = SL - SM, sRm = SR - SM;
   const float isLm = frcp(sLm), isRm = frcp(sRm);
   const float rLs = rL * sLu * isLm;
@@ -183,19 +179,10 @@ __device__ __forceinline__ Flux7 hlld(float gamma, float inv_gm1, float bx,
   const float b1Ls = b1L * kbL, b2Ls = b2L * kbL;
   const float b1Rs = b1R * kbR, b2Rs = b2R * kbR;
 
-  const float vbL = unL * bx + ut1L * b1L + ut2L * b2L;
-  const float vbR = unR * bx + ut1R * b1R + ut2R * b2R;
-  const float vbLs = SM * bx + u1Ls * b1Ls + u2Ls * b2Ls;
-  const float vbRs = SM * bx + u1Rs * b1Rs + u2Rs * b2Rs;
-  const float eLs = (sLu * eL - ptL * unL + pts * SM + bx * (vbL - vbLs)) * isLm;
-  const float eRs = (sRu * eR - ptR * unR + pts * SM + bx * (vbR - vbRs)) * isRm;
-
   // double-star (rotational) states; sqrt(rho*) and 1/sqrt(rho*) from one
   // reciprocal square root each
   const float isrL = __builtin_amdgcn_rsqf(rLs), isrR = __builtin_amdgcn_rsqf(rRs);
   const float srL = rLs * isrL, srR = rRs * isrR;
-  const float SLs = SM - abx * isrL;
-  const float SRs = SM + abx * isrR;
   const float f = frcp(srL + srR);
   const float u1ss = f * (srL * u1Ls + srR * u1Rs + (b1Rs - b1Ls) * sgn);
   const float u2ss = f * (srL * u2Ls + srR * u2Rs + (b2Rs - b2Ls) * sgn);
@@ -203,22 +190,31 @@ __device__ __forceinline__ Flux7 hlld(float gamma, float inv_gm1, float bx,
   const float b1ss = f * (srL * b1Rs + srR * b1Ls + srLR * (u1Rs - u1Ls) * sgn);
   const float b2ss = f * (srL * b2Rs + srR * b2Ls + srLR * (u2Rs - u2Ls) * sgn);
   const float vbss = SM * bx + u1ss * b1ss + u2ss * b2ss;
-  const float eLss = eLs - srL * (vbLs - vbss) * sgn;
-  const float eRss = eRs + srR * (vbRs - vbss) * sgn;
 
   // Select the fan region; every region's flux is F_K + sum of jumps.
   const bool left = SM >= 0.0f;  // regions 1-3 use the left side
-  // outer state on the chosen side
+  // outer and single-star state on the chosen side
   const float r = left ? rL : rR, un = left ? unL : unR;
   const float u1 = left ? ut1L : ut1R, u2 = left ? ut2L : ut2R;
   const float b1 = left ? b1L : b1R, b2 = left ? b2L : b2R;
-  const float e = left ? eL : eR, pt = left ? ptL : ptR, vb = left ? vbL : vbR;
-  const float S = left ? SL : SR;
-  const float Ss = left ? SLs : SRs;
+  const float p = left ? pL : pR, B2 = left ? B2L : B2R, pt = left ? ptL : ptR;
+  const float S = left ? SL : SR, sKu = left ? sLu : sRu, isKm = left ? isLm : isRm;
   const float rs = left ? rLs : rRs;
   const float u1s = left ? u1Ls : u1Rs, u2s = left ? u2Ls : u2Rs;
   const float b1s = left ? b1Ls : b1Rs, b2s = left ? b2Ls : b2Rs;
-  const float es = left ? eLs : eRs, ess = left ? eLss : eRss;
+  // the chosen side's energy chain: E, E* and E** (the other side's is not
+  // needed by any region)
+  const float v2 = un * un + u1 * u1 + u2 * u2;
+  const float kin = r * v2 + B2;
+  const float e = p * inv_gm1 + 0.5f * kin;
+  const float vb = un * bx + u1 * b1 + u2 * b2;
+  const float vbs = SM * bx + u1s * b1s + u2s * b2s;
+  const float es = (sKu * e - pt * un + pts * SM + bx * (vb - vbs)) * isKm;
+  // S*_K = SM -+ |Bx| / sqrt(rho*_K); E**_K = E*_K -+ sqrt(rho*_K) (v*.B* -
+  // v**.B**) sgn(Bx), - on the left, + on the right
+  const float sabx = left ? -abx : abx, ssr = left ? -srL : srR;
+  const float Ss = SM + sabx * (left ? isrL : isrR);
+  const float ess = es + ssr * (vbs - vbss) * sgn;
 
   Flux7 F;
   F.rho = r * un;
@@ -232,9 +228,9 @@ __device__ __forceinline__ Flux7 hlld(float gamma, float inv_gm1, float bx,
   // region: 0 outer, 1 single star, 2 double star
   int region;
   if (left)
-    region = SL > 0.0f ? 0 : (0.0f <= SLs ? 1 : 2);
+    region = SL > 0.0f ? 0 : (0.0f <= Ss ? 1 : 2);
   else
-    region = SR < 0.0f ? 0 : (SRs < 0.0f ? 1 : 2);
+    region = SR < 0.0f ? 0 : (Ss < 0.0f ? 1 : 2);
   if (region >= 1) {
     // F_K + S (U*_K - U_K)
     F.rho += S * (rs - r);

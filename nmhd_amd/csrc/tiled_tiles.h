@@ -137,8 +137,10 @@ struct FacesShape {
 // than it owns, spread over the first TY+TX threads), keeps their hydro
 // fluxes in LDS and differences them; the z-faces come from the register
 // queue, one plane of faces beyond zend.  Only the tile's own faces write
-// their B fluxes to memory.
-template <int TX, int TY>
+// their B fluxes to memory.  Recon is the box's reconstruction (b.recon: 0
+// minbee2, 1 piecewise constant) as a compile-time constant, so that the
+// scheme costs no per-variable selects; faces_tile_any dispatches on it.
+template <int TX, int TY, int Recon>
 __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, const FluxSet &Fs,
                                            float h, int x0, int y0, int z0, int zend,
                                            float *lds) {
@@ -152,7 +154,7 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
   const int plane = nx * ny;
   const int x = x0 + tx, y = y0 + ty;
   const bool valid = (x < nx) && (y < ny);
-  const int mode = b.recon;
+  constexpr int mode = Recon;
   const bool src = b.source == 1;
   const float idx = b.inv_d[0], idy = b.inv_d[1], idz = b.inv_d[2];
 
@@ -401,6 +403,18 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
     if (prefetch) store_plane(cb);  // plane p is no longer needed
     __syncthreads();
   }
+}
+
+// faces_tile for the box's reconstruction (a uniform branch: two copies of
+// the march, each without scheme selects).
+template <int TX, int TY>
+__device__ __forceinline__ void faces_tile_any(const Box &b, const FieldPtrs &W,
+                                               const FluxSet &Fs, float h, int x0, int y0,
+                                               int z0, int zend, float *lds) {
+  if (b.recon == 1)
+    faces_tile<TX, TY, 1>(b, W, Fs, h, x0, y0, z0, zend, lds);
+  else
+    faces_tile<TX, TY, 0>(b, W, Fs, h, x0, y0, z0, zend, lds);
 }
 
 template <int TX, int TY>
