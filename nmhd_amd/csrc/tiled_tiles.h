@@ -242,6 +242,49 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
     return hlld(b.gamma, b.inv_gm1, BY[c], L[0], Rr[0], L[1], Rr[1], L[2], Rr[2], L[3], Rr[3],
                 L[4], Rr[4], L[5], Rr[5], L[6], Rr[6]);
   };
+  // the lower y-face (isy) or x-face of the cell at LDS index c, with the
+  // stencil direction chosen per lane: the same expressions, in the same
+  // order, as face_y / face_x, so the two give the same bits.  The five
+  // hydro variables are reconstructed in the fixed order (rho, vx, vy, vz, p)
+  // and the velocities then permuted into (normal, t1, t2); of the
+  // transverse B averages, the one along z (B_z of this and the next plane)
+  // is common to both and the other (B_y along y for an x-face, B_x along x
+  // for a y-face) is read through a per-lane pointer.
+  auto face_xy = [&](int cb, int nb, int c, bool isy, int yrow) -> Flux7 {
+    const float *R = lds + (cb * 8 + 0) * LN, *VX = lds + (cb * 8 + 1) * LN;
+    const float *VY = lds + (cb * 8 + 2) * LN, *VZ = lds + (cb * 8 + 3) * LN;
+    const float *PR = lds + (cb * 8 + 4) * LN, *BX = lds + (cb * 8 + 5) * LN;
+    const float *BY = lds + (cb * 8 + 6) * LN, *BZ = lds + (cb * 8 + 7) * LN;
+    const float *BZN = lds + (nb * 8 + 7) * LN;
+    const float *OA = isy ? BX : BY, *OB = isy ? BX + 1 : BY + LX;
+    const int st = isy ? LX : 1;
+    const int s0 = c - 2 * st, s1 = c - st, s2 = c, s3 = c + st;
+    float L[7], Rr[7], lx, rx, ly, ry, lz, rz, lo, ro, lc, rc;
+    recon(mode, R[s0], R[s1], R[s2], R[s3], L[0], Rr[0]);
+    recon(mode, VX[s0], VX[s1], VX[s2], VX[s3], lx, rx);
+    recon(mode, VY[s0], VY[s1], VY[s2], VY[s3], ly, ry);
+    recon(mode, VZ[s0], VZ[s1], VZ[s2], VZ[s3], lz, rz);
+    recon(mode, PR[s0], PR[s1], PR[s2], PR[s3], L[4], Rr[4]);
+    recon(mode, 0.5f * (OA[s0] + OB[s0]), 0.5f * (OA[s1] + OB[s1]), 0.5f * (OA[s2] + OB[s2]),
+          0.5f * (OA[s3] + OB[s3]), lo, ro);
+    recon(mode, 0.5f * (BZ[s0] + BZN[s0]), 0.5f * (BZ[s1] + BZN[s1]), 0.5f * (BZ[s2] + BZN[s2]),
+          0.5f * (BZ[s3] + BZN[s3]), lc, rc);
+    // x-face: (vx, vy, vz), (B_y, B_z); y-face: (vy, vz, vx), (B_z, B_x)
+    L[1] = isy ? ly : lx, Rr[1] = isy ? ry : rx;
+    L[2] = isy ? lz : ly, Rr[2] = isy ? rz : ry;
+    L[3] = isy ? lx : lz, Rr[3] = isy ? rx : rz;
+    L[5] = isy ? lc : lo, Rr[5] = isy ? rc : ro;
+    L[6] = isy ? lo : lc, Rr[6] = isy ? ro : rc;
+    if (src) {
+      const float *VA = isy ? VX : VY;
+      const float va = 0.5f * (VA[s1] + VA[s2]);
+      const float mbv = 0.5f * h * (isy ? accel_y(b, va, yrow) : accel_x(b, va));
+      L[1] += mbv;
+      Rr[1] += mbv;
+    }
+    return hlld(b.gamma, b.inv_gm1, isy ? BY[c] : BX[c], L[0], Rr[0], L[1], Rr[1], L[2], Rr[2],
+                L[3], Rr[3], L[4], Rr[4], L[5], Rr[5], L[6], Rr[6]);
+  };
   // hydro fluxes in the conserved order (rho, m_x, m_y, m_z, E)
   auto put_x = [&](int row, int col, const Flux7 &r) {
     float *d = FXL + row * (TX + 1) + col;
@@ -325,18 +368,37 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
           put_nt(&SB.fb[3][a], r.bt2);  // B_x
         }
       }
-      // the tile's upper boundary faces: the x-faces of column x0+TX on the
-      // first TY lanes of one wave, the y-faces of row y0+TY on the first TX
-      // lanes of another; the two waves rotate with the plane so that the
-      // SIMDs of a CU share the extra HLLD evaluations evenly
+      // the tile's upper boundary faces, the y-faces of row y0+TY and the
+      // x-faces of column x0+TX, on a wave that rotates with the plane and
+      // starts at an offset given by the tile's position, so that the SIMDs
+      // of a CU share the extra HLLD evaluations evenly also when the
+      // workgroups on it march in step: when both fit one wave (TX + TY <=
+      // 64), in one pass (lanes [0, TX) the y-faces, [TX, TX+TY) the
+      // x-faces), else on two waves
       const int lane = tid & 63, wave = tid >> 6;
-      if (wave == ((2 * p) & 3) && lane < TY) {
-        const int yy = y0 + lane;
-        if (yy < ny && x0 + TX <= nx) put_x(lane, TX, face_x(cb, nb, (lane + 2) * LX + TX + 2));
-      } else if (wave == ((2 * p + 1) & 3) && lane < TX) {
-        const int yy = y0 + TY;
-        if (x0 + lane < nx && yy <= ny)
-          put_y(TY, lane, face_y(cb, nb, (TY + 2) * LX + lane + 2, yy < ny ? yy : yy - ny));
+      const int rot = p + x0 / TX + y0 / TY;
+      if constexpr (TX + TY <= 64) {
+        if (wave == (rot & 3) && lane < TX + TY) {
+          const bool isy = lane < TX;
+          const int row = isy ? TY : lane - TX, col = isy ? lane : TX;
+          const int yy = y0 + row;
+          if (isy ? (x0 + lane < nx && yy <= ny) : (yy < ny && x0 + TX <= nx)) {
+            const Flux7 r = face_xy(cb, nb, (row + 2) * LX + col + 2, isy, yy < ny ? yy : yy - ny);
+            if (isy)
+              put_y(TY, lane, r);
+            else
+              put_x(row, TX, r);
+          }
+        }
+      } else {
+        if (wave == ((2 * rot) & 3) && lane < TY) {
+          const int yy = y0 + lane;
+          if (yy < ny && x0 + TX <= nx) put_x(lane, TX, face_x(cb, nb, (lane + 2) * LX + TX + 2));
+        } else if (wave == ((2 * rot + 1) & 3) && lane < TX) {
+          const int yy = y0 + TY;
+          if (x0 + lane < nx && yy <= ny)
+            put_y(TY, lane, face_y(cb, nb, (TY + 2) * LX + lane + 2, yy < ny ? yy : yy - ny));
+        }
       }
     }
     // z-face below plane p-1 (one plane beyond the chunk: the upper face of
