@@ -111,7 +111,9 @@ typedef struct nmhd_geometry {
 typedef struct nmhd_ctx nmhd_ctx;
 
 /* MHDSolver::MHDSolver (INL:14-41): allocate the device state for one box on
- * the current HIP device. */
+ * the current HIP device.  Limits: fewer than 2^31 cells including ghosts,
+ * and fewer than 2^30 in one XY plane (the kernels index with 32-bit
+ * offsets). */
 int nmhd_create(const nmhd_desc *desc, nmhd_ctx **out);
 /* ~MHDSolver: free the context and its device memory. */
 int nmhd_destroy(nmhd_ctx *ctx);

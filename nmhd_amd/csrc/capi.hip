@@ -45,6 +45,10 @@ bool desc_valid(const nmhd_desc *d, std::string &why) {
   long n = 1;
   for (int i = 0; i < 3; ++i) n *= (long)(d->global_size[i] + 2 * d->margin[i]);
   if (n >= (1L << 31)) return why = "box exceeds 2^31 cells", false;
+  // the kernels address a plane with 32-bit byte offsets
+  if ((long)(d->global_size[0] + 2 * d->margin[0]) * (d->global_size[1] + 2 * d->margin[1]) >=
+      (1L << 30))
+    return why = "an XY plane exceeds 2^30 cells", false;
   return true;
 }
 

@@ -34,6 +34,7 @@
 #pragma once
 
 #include "nmhd_kernels.h"
+#include <type_traits>
 
 namespace nmhd {
 namespace tiled {
@@ -55,6 +56,25 @@ __device__ __forceinline__ int pmod(int i, int n) {
 __device__ __forceinline__ void put_nt(float *dst, float v) { __builtin_nontemporal_store(v, dst); }
 // Likewise for values that only the loading thread reads (no halo reuse).
 __device__ __forceinline__ float get_nt(const float *src) { return __builtin_nontemporal_load(src); }
+
+// The element at byte offset boff (32-bit) from a workgroup-uniform base:
+// loads and stores through it take the scalar-base + 32-bit vector-offset
+// form, with no 64-bit address arithmetic per lane.  The offsets used are
+// in-plane (an XY plane holds fewer than 2^30 cells, checked at creation).
+template <class T>
+__device__ __forceinline__ T *at_byte(T *base, unsigned boff) {
+  using C = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+  return reinterpret_cast<T *>(reinterpret_cast<C *>(base) + boff);
+}
+
+// A per-lane offset made opaque to the optimiser for the current plane (an
+// empty asm operand, no instruction): otherwise the loop-invariant sum of
+// an array base and the offset is hoisted out of the plane loop as a 64-bit
+// per-lane pointer per array, which costs registers and 64-bit adds.
+__device__ __forceinline__ unsigned opaque(unsigned v) {
+  asm volatile("" : "+v"(v));
+  return v;
+}
 
 __device__ __forceinline__ void recon(int mode, float x0, float x1, float x2, float x3, float &l,
                                       float &r) {
@@ -158,24 +178,27 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
   const bool src = b.source == 1;
   const float idx = b.inv_d[0], idy = b.inv_d[1], idz = b.inv_d[2];
 
-  // global in-plane offset of every LDS slot this thread stages (each thread
-  // writes and reads only its own slots)
-  int *const GXY = reinterpret_cast<int *>(lds + S::kGxy);
+  // global in-plane byte offset of every LDS slot this thread stages (each
+  // thread writes and reads only its own slots)
+  unsigned *const GXY = reinterpret_cast<unsigned *>(lds + S::kGxy);
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
     const int li = tid + s * NT;
     const int lx = li % LX, ly = li / LX;
-    if (li < LN) GXY[li] = pmod(y0 - 2 + ly, ny) * nx + pmod(x0 - 2 + lx, nx);
+    if (li < LN) GXY[li] = (unsigned)(pmod(y0 - 2 + ly, ny) * nx + pmod(x0 - 2 + lx, nx)) * 4u;
   }
+  // this thread's own cell: in-plane byte offset
+  const unsigned own = (unsigned)(y * nx + x) * 4u;
+  // start of plane p (wrapped) in an array
+  auto plane_of = [&](auto *arr, int p) { return arr + (size_t)wrap(p, nz) * plane; };
   float stage[NS][8];
   auto load_plane = [&](int p) {
-    const int base = wrap(p, nz) * plane;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
       if (tid + s * NT < LN) {
-        const int g = base + GXY[tid + s * NT];
+        const unsigned g = opaque(GXY[tid + s * NT]);
 #pragma unroll
-        for (int f = 0; f < 8; ++f) stage[s][f] = W.f[f][g];
+        for (int f = 0; f < 8; ++f) stage[s][f] = *at_byte(plane_of(W.f[f], p), g);
       }
     }
   };
@@ -329,6 +352,7 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
   for (int p = zs; p <= zlast; ++p) {
     const int cb = p & 1, nb = (p + 1) & 1;
     const bool prefetch = (p + 2 <= zlast);
+    const unsigned ownp = opaque(own);
     if (prefetch) load_plane(p + 2);
     {
       const float *P0 = lds + cb * 8 * LN;
@@ -354,18 +378,16 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
         const Flux7 r = face_x(cb, nb, c);
         put_x(ty, tx, r);
         if (valid) {
-          const int a = wrap(p, nz) * plane + y * nx + x;
-          put_nt(&SB.fb[0][a], r.bt1);  // B_y
-          put_nt(&SB.fb[1][a], r.bt2);  // B_z
+          put_nt(at_byte(plane_of(SB.fb[0], p), ownp), r.bt1);  // B_y
+          put_nt(at_byte(plane_of(SB.fb[1], p), ownp), r.bt2);  // B_z
         }
       }
       if (x < nx && y <= ny) {
         const Flux7 r = face_y(cb, nb, c, yrow);
         put_y(ty, tx, r);
         if (valid) {
-          const int a = wrap(p, nz) * plane + y * nx + x;
-          put_nt(&SB.fb[2][a], r.bt1);  // B_z
-          put_nt(&SB.fb[3][a], r.bt2);  // B_x
+          put_nt(at_byte(plane_of(SB.fb[2], p), ownp), r.bt1);  // B_z
+          put_nt(at_byte(plane_of(SB.fb[3], p), ownp), r.bt2);  // B_x
         }
       }
       // the tile's upper boundary faces, the y-faces of row y0+TY and the
@@ -429,9 +451,8 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
       fzn[3] = r.mn;
       fzn[4] = r.e;
       if (p - 1 < zend) {
-        const int a = wrap(p - 1, nz) * plane + y * nx + x;
-        put_nt(&SB.fb[4][a], r.bt1);  // B_x
-        put_nt(&SB.fb[5][a], r.bt2);  // B_y
+        put_nt(at_byte(plane_of(SB.fb[4], p - 1), ownp), r.bt1);  // B_x
+        put_nt(at_byte(plane_of(SB.fb[5], p - 1), ownp), r.bt2);  // B_y
       }
     }
     __syncthreads();
@@ -450,9 +471,9 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
     }
     // plane p-2 complete: its lower z-face (fzl) and upper z-face (fzn)
     if (valid && p - 2 >= z0 && p - 2 < zend) {
-      const int a = wrap(p - 2, nz) * plane + y * nx + x;
 #pragma unroll
-      for (int k = 0; k < 5; ++k) put_nt(&SB.pace[k][a], pxy2[k] + (fzl[k] - fzn[k]) * idz);
+      for (int k = 0; k < 5; ++k)
+        put_nt(at_byte(plane_of(SB.pace[k], p - 2), ownp), pxy2[k] + (fzl[k] - fzn[k]) * idz);
     }
 #pragma unroll
     for (int k = 0; k < 5; ++k) {

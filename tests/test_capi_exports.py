@@ -74,6 +74,8 @@ def _bad_desc_cases():
     case("margin beyond interior", lambda d: d.margin.__setitem__(1, 9), b"margin[1]")
     case("2^31 cells", lambda d: [d.global_size.__setitem__(i, 1291) for i in range(3)],
          b"2^31")
+    case("2^30-cell plane", lambda d: [d.global_size.__setitem__(i, v)
+                                       for i, v in enumerate((32768, 32768, 1))], b"2^30")
     return base, cases
 
 
