@@ -378,11 +378,22 @@ def main():
     if "valu_insts" in prof:
         # the bound that binds this kernel: its VALU issue rate (instruction
         # count per launch from the profile, time live) against the
-        # non-packed FP32 peak
+        # non-packed FP32 peak.  The count belongs to the kernel the profile
+        # traced; a live time far from the traced one means the profile is
+        # of another build, and its count is not used.
         lane_ops = prof["valu_insts"] * 64.0
-        roofline["valu"] = {"lane_ops_per_launch": lane_ops,
-                            "achieved": lane_ops / mean_s / 1e12, "peak": VALU_PEAK / 1e12,
-                            "unit": "Tlane-op/s", "frac": lane_ops / mean_s / VALU_PEAK}
+        traced = prof.get("trace_mean_us")
+        if traced and abs(mean_s * 1e6 - traced) > 0.05 * traced:
+            roofline["valu"] = None
+            roofline["valu_note"] = ("committed profile traced this kernel at %.1f us, "
+                                     "measured now %.1f us: count not used"
+                                     % (traced, mean_s * 1e6))
+        else:
+            roofline["valu"] = {"lane_ops_per_launch": lane_ops,
+                                "achieved": lane_ops / mean_s / 1e12,
+                                "peak": VALU_PEAK / 1e12, "unit": "Tlane-op/s",
+                                "frac": lane_ops / mean_s / VALU_PEAK,
+                                "profile_trace_mean_us": traced}
     step_achieved = value * STEP_BYTES
     out = {
         "metric": "zone-updates/s at 256^3 fp32, % HBM roofline; batched & adjoint vs plain",

@@ -82,8 +82,15 @@ def main():
     a = ap.parse_args()
     if a.traffic_json:
         import json
+        kern = traffic(a.fetch, a.write, a.fetch_scale, a.write_scale, a.valu)
+        # the kernel-trace mean duration beside the counts: a reader (and the
+        # bench) can tell whether the counts belong to the kernel being timed
+        times = kernel_times(a.trace)
+        for r in kern.values():
+            if r["rocprof_kernel"] in times:
+                r["trace_mean_us"] = times[r["rocprof_kernel"]][1]
         rec = {"global_size": list(a.global_size), "path": a.path, "command": a.command,
-               "kernels": traffic(a.fetch, a.write, a.fetch_scale, a.write_scale, a.valu)}
+               "kernels": kern}
         with open(a.traffic_json, "w") as fh:
             json.dump(rec, fh, indent=1)
         return
