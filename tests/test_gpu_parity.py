@@ -280,6 +280,73 @@ def test_tiled_odd_shapes(gpu_ready, shape):
     assert group_rel_err(a, b).max() < 1e-3
 
 
+def rough_state(cfg, seed=21):
+    """rho, p in [0.5, 1.5], v in [-1, 1], B in [-0.7, 0.7] independently
+    per cell from the counter RNG, and B = 0 on about 1/16 of the faces of
+    each direction (signBx = 0): neighbouring cells differ strongly, so the
+    HLLD takes its single- and double-star regions on both sides."""
+    n = cfg["n"]
+    idx = np.arange(n[0] * n[1] * n[2], dtype=np.uint64)
+    lo = [0.5, -1.0, -1.0, -1.0, 0.5, -0.7, -0.7, -0.7]
+    wd = [1.0, 2.0, 2.0, 2.0, 1.0, 1.4, 1.4, 1.4]
+    W = np.empty((8, n[2], n[1], n[0]), np.float32)
+    for c in range(8):
+        W[c] = (lo[c] + wd[c] * P.uniform01(seed, idx, c)).reshape(n[2], n[1], n[0])
+    for c in (5, 6, 7):
+        zero = (P.uniform01(seed + 1, idx, c) < 1.0 / 16.0).reshape(n[2], n[1], n[0])
+        W[c][zero] = 0.0
+    return W
+
+
+@pytest.mark.parametrize("shape,shear", [((96, 80, 40), False), ((70, 37, 20), False),
+                                         ((64, 48, 24), True)])
+def test_tiled_tile_shapes_bit_identical(gpu_ready, shape, shear):
+    """Every pair of face / update tile shapes gives the same step bit for
+    bit (the tiles' boundary faces, evaluated in one pass or two, and their
+    rotation over the waves do not change a cell's arithmetic), on rough
+    states, boxes that the tiles do not divide, and a shearing box."""
+    announce("tile shapes bit-identical, %s%s" % (shape, " shearing" if shear else ""))
+    if shear:
+        cfg = P.make_config(shape, margin=(0, 4, 0), cfl=0.3, source=1, shear_y=1)
+    else:
+        cfg = P.make_config(shape, cfl=0.3)
+    W = rough_state(cfg)
+    ref = None
+    for ft in (capi.TILE_32x8, capi.TILE_64x4, capi.TILE_16x16):
+        for ut in (capi.TILE_64x4, capi.TILE_32x8, capi.TILE_16x16):
+            g = capi.Context(cfg, path=capi.PATH_TILED)
+            g.set_tiles(ft, ut)
+            g.set_state(W)
+            g.step()
+            out = (g.get_state(), g.get_time())
+            g.close()
+            if ref is None:
+                ref = out
+                assert np.isfinite(ref[0]).all()
+                continue
+            assert out[1] == ref[1], (ft, ut)
+            assert bitwise_equal(out[0], ref[0]), (ft, ut)
+
+
+@pytest.mark.parametrize("shape,shear", [((40, 24, 12), False), ((24, 16, 8), True)])
+def test_strict_rough_state_bitwise(gpu_ready, shape, shear):
+    """The strict path equals the oracle bit for bit on rough states too
+    (single- and double-star regions on both sides, B_n = 0 faces), after
+    two steps."""
+    announce("strict rough state %s%s" % (shape, " shearing" if shear else ""))
+    if shear:
+        cfg = P.make_config(shape, margin=(0, 4, 0), cfl=0.3, source=1, shear_y=1)
+    else:
+        cfg = P.make_config(shape, cfl=0.3)
+    o, g = both(cfg, rough_state(cfg, seed=22), path=capi.PATH_STRICT)
+    for _ in range(2):
+        dt_o, _ = o.step()
+        dt_g, _ = g.step()
+        assert dt_g == dt_o
+    assert g.get_time()[0] == o.time
+    assert bitwise_equal(g.get_state(), o.get_state())
+
+
 # ------------------------------------------------------------ stepping
 
 def test_goal_clipping(gpu_ready):
