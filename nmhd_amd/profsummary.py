@@ -13,6 +13,7 @@ import sqlite3
 
 
 def short(name):
+    name = name.replace("(anonymous namespace)::", "")
     name = name.split("(")[0]
     return name.replace("void ", "")
 
