@@ -13,6 +13,10 @@
  * FP64-promoted literals as the reference; build with -ffp-contract=off so
  * that no fused multiply-add changes the rounding.
  *
+ * Each function is pinned by the checks of tests/test_oracle_*.py (golden
+ * HLLD vectors from the reference's own hlld.inl, closed forms, conservation,
+ * symmetries, convergence against exact solutions); none is left unpinned.
+ *
  * Data layout (identical to the reference, crystal.h:31-50): one array per
  * field, addr = (z*ny + y)*nx + x, x fastest; face-centred B_d[i] sits on the
  * lower d-face of cell i; every axis wraps periodically over the whole array
