@@ -1,0 +1,89 @@
+"""Physics of the shearing box at realistic sizes, where no oracle runs:
+what the equations and the scheme's symmetries fix, on the default (tiled)
+path.
+
+* The magneto-rotational instability.  For Keplerian shear (q = 1.5) the
+  ideal MRI of a vertical field grows at most at sigma = q Omega / 2 = 0.75
+  Omega, so the perturbation's magnetic energy grows at most at 1.5 Omega.
+  The 64^3 box with beta 400 resolves the fastest-growing wavelength
+  (2 pi v_A / Omega * sqrt(16/15) = 0.46, about 29 cells): from t = 13 to 15
+  the energy <B_az^2> + <B_rad^2> must grow at a rate between 1.1 and 1.5
+  Omega, with the azimuthal and radial parts comparable (the channel mode),
+  div B at round-off and the mean vertical field nearly unchanged (it moves
+  only through the EMFs at the radial boundaries, which the scheme takes
+  from ghost rows that its first stage evolves on their own).  A sign error
+  in the Coriolis or tidal term, the shear remap or the constrained
+  transport stops or changes the growth.
+* The unperturbed shear flow (rho, p, B_z uniform, v_az = -q Omega x_rad)
+  at 256^3: every operation of the step commutes with shifts along the
+  azimuth and the vertical, so the state stays uniform along both bit for
+  bit after 200 steps (a tile edge, halo or boundary-face slip breaks
+  that), and away from the radial boundaries it stays at the equilibrium
+  to round-off.  (Over longer times the reference's scheme itself drifts
+  from it by a uniform epicyclic velocity, ~1e-4 after t = 1.5 at 64^3, as
+  the oracle shows; the ghost rows, which its first stage evolves with
+  stencils wrapping across the array, depart more.)
+Radial axis Y, azimuth X, as in the reference.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from nmhd_amd import capi, problems as P
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(msg):
+    print("\n[case] " + msg, flush=True)
+
+
+def test_mri_growth_rate(gpu_ready):
+    announce("shearing box 64^3, beta 400: MRI growth to t = 15")
+    cfg = P.config_c2((64, 64, 64))
+    g = capi.Context(cfg)
+    g.set_state(P.shearing_box_mri(cfg, seed=1002))
+    rec = {}
+    for goal in (13.0, 14.0, 15.0):
+        g.run(goal, 1000000)
+        t = g.get_time()[0]
+        assert abs(t - goal) < 1e-6
+        rec[goal] = g.diagnostics()
+    g.close()
+    e = {k: v["b2_mean"][0] + v["b2_mean"][1] for k, v in rec.items()}
+    rates = [math.log(e[14.0] / e[13.0]), math.log(e[15.0] / e[14.0])]
+    bx2, by2, bz2 = rec[15.0]["b2_mean"]
+    print("energy growth rates %s; <Bx^2> %.3g <By^2> %.3g <Bz^2> %.3g; div B %.2g"
+          % (np.round(rates, 3), bx2, by2, bz2, rec[15.0]["divb_max"]))
+    assert all(1.1 <= r <= 1.5 for r in rates), rates
+    assert 0.7 <= bx2 / by2 <= 1.4
+    # the mean B_z (total vertical flux) is conserved
+    b0 = math.sqrt(2.0 / 400.0)
+    nz_total = rec[15.0]["total"][7] / (64 ** 3)
+    print("mean B_z %.9g (initial %.9g), relative change %.3g" % (nz_total, b0, nz_total / b0 - 1))
+    assert abs(nz_total - b0) < 2e-3 * b0
+    assert rec[15.0]["divb_max"] < 1e-4 * b0
+
+
+def test_shear_flow_symmetry_and_uniform_drift(gpu_ready):
+    announce("shearing box 256^3 without perturbation: 200 steps")
+    cfg = P.config_c2((256, 256, 256))
+    W0 = P.shearing_box_mri(cfg, seed=1002, amp=0.0)
+    g = capi.Context(cfg)
+    g.set_state(W0)
+    k, _ = g.run(1e30, 200)
+    assert k == 200
+    W = g.get_state()
+    g.close()
+    # uniform along x (azimuth) and z (vertical), bit for bit
+    assert np.array_equal(W, np.broadcast_to(W[:, :, :, :1], W.shape))
+    assert np.array_equal(W, np.broadcast_to(W[:, :1, :, :], W.shape))
+    m, ny = cfg["m"][1], cfg["n"][1]
+    rows = slice(m + 4, ny - m - 4)  # interior rows away from the radial boundaries
+    d = W[:, 0, rows, 0].astype(np.float64) - W0[:, 0, rows, 0]
+    print("max departure per field, interior rows: %s"
+          % np.array2string(np.abs(d).max(axis=1), precision=2))
+    # over these 200 steps (t = 0.15) the departure is at round-off for
+    # values of order one (v_az up to 0.75: ulp 6e-8, ~20 ulp accumulated)
+    assert np.abs(d).max() < 5e-6
