@@ -15,7 +15,8 @@ The JSON line also carries
                   against the 8 TB/s HBM peak;
   step_roofline -- the whole step: 160 algorithmic bytes per zone-update;
   cpu_baseline -- the oracle (CPU restatement of the reference step) on the
-                  GPU box's host cores, a 2-step sample of the same box;
+                  host cores of the machine that runs the bench, a 2-step
+                  sample of the same box;
   batched      -- 64 periodic boxes of uneven sizes (edges from
                   {16,24,32,48,64,96}, seed 1004; Orszag-Tang and blast
                   states) advanced 100 steps with per-box dt by the batched

@@ -5,7 +5,8 @@
 * ``_ref/libref_hlld.so`` -- the reference's own ``library/hlld.inl`` compiled
   for the host from the reference tree (``$NMHD_REF`` or /root/reference),
   used to pin the oracle's HLLD bit for bit.  Skipped when the tree is absent
-  (the GPU box never has it; the built file travels with the repository).
+  (a machine without it uses the built file, which travels with the
+  repository's working tree).
 
 Run ``python oracle/build.py`` or call :func:`build`.
 """
