@@ -61,6 +61,10 @@ struct nmhd_batch {
   WorkItem *d_fitems = nullptr, *d_uitems = nullptr, *d_titems = nullptr;
   int nf = 0, nu = 0;
   int nfv[TV_COUNT] = {}, nuv[TV_COUNT] = {};  // items per tile shape (the lists are grouped)
+  // side streams and events that let the launches of different tile shapes
+  // run concurrently (forked from and joined back into stream)
+  hipStream_t side[TV_COUNT - 1] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[TV_COUNT - 1] = {};
   size_t cap_f = 0, cap_u = 0, cap_t = 0;
   bool items_dirty = true;
   bool cfl_valid = false;
@@ -160,57 +164,86 @@ int grid_of(int nitems, int wg_per_cu, int cus) {
   return std::max(1, std::min(nitems, wg_per_cu * cus));
 }
 
+// One launch per tile shape that has items, launch(stream, shape, offset):
+// the first on s, the others on side streams forked from s and joined back
+// into it.  The shapes' items belong to different boxes, so the launches are
+// independent, and running them side by side fills the GPU where a small
+// launch alone would not (and hides each launch's tail).  Works eagerly and
+// under graph capture.
+template <class F>
+int over_shapes(nmhd_batch *b, hipStream_t s, const int *counts, F &&launch) {
+  int used[TV_COUNT], off[TV_COUNT], nused = 0;
+  for (int v = 0, o = 0; v < TV_COUNT; o += counts[v], ++v)
+    if (counts[v] > 0) used[nused] = v, off[nused++] = o;
+  if (nused == 0) return NMHD_OK;
+  if (nused > 1) BHIP(b, hipEventRecord(b->ev_fork, s));
+  launch(s, used[0], off[0]);
+  for (int k = 1; k < nused; ++k) {
+    hipStream_t t = b->side[k - 1];
+    BHIP(b, hipStreamWaitEvent(t, b->ev_fork, 0));
+    launch(t, used[k], off[k]);
+    BHIP(b, hipEventRecord(b->ev_join[k - 1], t));
+    BHIP(b, hipStreamWaitEvent(s, b->ev_join[k - 1], 0));
+  }
+  return NMHD_OK;
+}
+
 // The face / update launches of one stage over a grouped item list: the
 // persistent kernel with wg workgroups per CU, or (wg == 0) one launch per
 // tile shape with one workgroup per item.
-void faces_over(nmhd_batch *b, hipStream_t s, int wg, const WorkItem *items, int n,
-                const int *counts, float *state, const StepState *st, int hsel) {
+int faces_over(nmhd_batch *b, hipStream_t s, int wg, const WorkItem *items, int n,
+               const int *counts, float *state, const StepState *st, int hsel) {
   if (wg > 0) {
     launch_batch_faces(s, grid_of(n, wg, b->cus), b->d_boxes, items, n, state, b->flux, st, hsel);
-    return;
+    return NMHD_OK;
   }
-  for (int v = 0, off = 0; v < TV_COUNT; off += counts[v], ++v)
-    launch_batch_faces_items(s, v, b->d_boxes, items + off, counts[v], state, b->flux, st, hsel);
+  return over_shapes(b, s, counts, [&](hipStream_t t, int v, int off) {
+    launch_batch_faces_items(t, v, b->d_boxes, items + off, counts[v], state, b->flux, st, hsel);
+  });
 }
 
-void update_over(nmhd_batch *b, hipStream_t s, int wg, const WorkItem *items, int n,
-                 const int *counts, float *in, float *base, float *out, StepState *st, int hsel,
-                 bool cfl) {
+int update_over(nmhd_batch *b, hipStream_t s, int wg, const WorkItem *items, int n,
+                const int *counts, float *in, float *base, float *out, StepState *st, int hsel,
+                bool cfl) {
   if (wg > 0) {
     launch_batch_update(s, grid_of(n, wg, b->cus), b->d_boxes, items, n, in, b->flux, base, out,
                         st, hsel, cfl);
-    return;
+    return NMHD_OK;
   }
-  for (int v = 0, off = 0; v < TV_COUNT; off += counts[v], ++v)
-    launch_batch_update_items(s, v, b->d_boxes, items + off, counts[v], in, b->flux, base, out,
+  return over_shapes(b, s, counts, [&](hipStream_t t, int v, int off) {
+    launch_batch_update_items(t, v, b->d_boxes, items + off, counts[v], in, b->flux, base, out,
                               st, hsel, cfl);
+  });
 }
 
 // One iteration of the proceed() loop for every box.
-void enqueue_step(nmhd_batch *b, bool full_cfl, float goal) {
+int enqueue_step(nmhd_batch *b, bool full_cfl, float goal) {
   hipStream_t s = b->stream;
   const int A = b->cur, n = b->nbox;
   launch_batch_begin(s, b->st, n, full_cfl);
   if (full_cfl) launch_batch_cfl(s, b->d_boxes, n, b->state[A], goal, b->st);
   launch_batch_dt(s, b->st, n, b->d_goal);
   auto stage = [&](int in, int base, int out, int hsel, bool last) {
-    faces_over(b, s, b->fwg, b->d_fitems, b->nf, b->nfv, b->state[in], b->st, hsel);
-    update_over(b, s, b->uwg, b->d_uitems, b->nu, b->nuv, b->state[in], b->state[base],
-                b->state[out], b->st, hsel, last);
+    const int r = faces_over(b, s, b->fwg, b->d_fitems, b->nf, b->nfv, b->state[in], b->st, hsel);
+    if (r) return r;
+    return update_over(b, s, b->uwg, b->d_uitems, b->nu, b->nuv, b->state[in], b->state[base],
+                       b->state[out], b->st, hsel, last);
   };
-  int next;
+  int next, r;
   if (b->integrator == NMHD_EULER1) {
     next = (A + 1) % 3;
-    stage(A, A, next, H_FULL, true);
+    r = stage(A, A, next, H_FULL, true);
   } else {
     const int B = (A + 1) % 3;
     next = (A + 2) % 3;
-    stage(A, A, B, H_HALF, false);
-    stage(B, A, next, H_FULL, true);
+    r = stage(A, A, B, H_HALF, false);
+    if (!r) r = stage(B, A, next, H_FULL, true);
   }
+  if (r) return r;
   launch_batch_copy(s, b->d_boxes, n, b->state[A], b->state[next], b->st);
   launch_batch_commit(s, b->st, n);
   b->cur = next;
+  return NMHD_OK;
 }
 
 int ensure_graph(nmhd_batch *b) {
@@ -218,9 +251,13 @@ int ensure_graph(nmhd_batch *b) {
   if (b->exec[phase]) return NMHD_OK;
   hipGraph_t g = nullptr;
   BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-  enqueue_step(b, false, 0.0f);
+  const int re = enqueue_step(b, false, 0.0f);
   const hipError_t e = hipStreamEndCapture(b->stream, &g);
   b->cur = phase;  // capture records; it does not advance the state
+  if (re) {
+    if (g) (void)hipGraphDestroy(g);
+    return re;
+  }
   if (e != hipSuccess)
     return bfail(b, NMHD_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
   const hipError_t ei = hipGraphInstantiate(&b->exec[phase], g, nullptr, nullptr, 0);
@@ -260,6 +297,11 @@ void free_batch(nmhd_batch *b) {
   b->d_goal = nullptr;
   b->d_boxes = nullptr;
   b->d_fitems = b->d_uitems = b->d_titems = nullptr;
+  for (auto &t : b->side)
+    if (t) (void)hipStreamDestroy(t), t = nullptr;
+  for (auto &e : b->ev_join)
+    if (e) (void)hipEventDestroy(e), e = nullptr;
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork), b->ev_fork = nullptr;
   if (b->stream) (void)hipStreamDestroy(b->stream), b->stream = nullptr;
 }
 
@@ -435,9 +477,14 @@ int tune(nmhd_batch *b, const char *path, int *timed) {
       for (int wg : kWgPerCu) {
         float ms = 0.0f;
         const int n = (int)fi.size();
+        int rl = NMHD_OK;
         r = time_launch(
-            b, [&] { faces_over(b, b->stream, wg, b->d_titems, n, cf, in, b->st_tune, H_HALF); },
+            b,
+            [&] {
+              if (!rl) rl = faces_over(b, b->stream, wg, b->d_titems, n, cf, in, b->st_tune, H_HALF);
+            },
             &ms);
+        if (!r) r = rl;
         if (r) return r;
         if (ms < best_f) best_f = ms, fzc = zc, fwg = wg;
       }
@@ -447,13 +494,16 @@ int tune(nmhd_batch *b, const char *path, int *timed) {
       for (int wg : kWgPerCu) {
         float ms = 0.0f;
         const int n = (int)ui.size();
+        int rl = NMHD_OK;
         r = time_launch(
             b,
             [&] {
-              update_over(b, b->stream, wg, b->d_titems, n, cu, in, in, scratch, b->st_tune,
-                          H_HALF, false);
+              if (!rl)
+                rl = update_over(b, b->stream, wg, b->d_titems, n, cu, in, in, scratch,
+                                 b->st_tune, H_HALF, false);
             },
             &ms);
+        if (!r) r = rl;
         if (r) return r;
         if (ms < best_u) best_u = ms, uzc = zc, uwg = wg;
       }
@@ -533,6 +583,14 @@ int nmhd_batch_create(const nmhd_desc *descs, int nbox, nmhd_batch **out) {
   if (colon != std::string::npos) b->arch = b->arch.substr(0, colon);
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(NMHD_EHIP, "hipStreamCreate failed");
+  for (auto &t : b->side)
+    if (hipStreamCreateWithFlags(&t, hipStreamNonBlocking) != hipSuccess)
+      return bail(NMHD_EHIP, "hipStreamCreate failed");
+  if (hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess)
+    return bail(NMHD_EHIP, "hipEventCreate failed");
+  for (auto &e : b->ev_join)
+    if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess)
+      return bail(NMHD_EHIP, "hipEventCreate failed");
   for (auto &p : b->state)
     if (hipMalloc(&p, sizeof(float) * (size_t)off8) != hipSuccess)
       return bail(NMHD_ENOMEM, "state pool allocation failed");
@@ -689,7 +747,8 @@ int nmhd_batch_run(nmhd_batch *b, float goal_time, int nsteps, int use_graph, in
                          b->stream));
   for (int k = 0; k < nsteps; ++k) {
     if (!b->cfl_valid) {
-      enqueue_step(b, true, goal_time);
+      r = enqueue_step(b, true, goal_time);
+      if (r) return r;
       b->cfl_valid = true;
     } else if (use_graph) {
       r = ensure_graph(b);
@@ -697,7 +756,8 @@ int nmhd_batch_run(nmhd_batch *b, float goal_time, int nsteps, int use_graph, in
       BHIP(b, hipGraphLaunch(b->exec[b->cur], b->stream));
       b->cur = b->integrator == NMHD_EULER1 ? (b->cur + 1) % 3 : (b->cur + 2) % 3;
     } else {
-      enqueue_step(b, false, goal_time);
+      r = enqueue_step(b, false, goal_time);
+      if (r) return r;
     }
   }
   BHIP(b, hipGetLastError());
