@@ -41,12 +41,16 @@ __device__ __forceinline__ long nb(const Box &b, int x, int y, int z, int d, int
   return cell_index(b, x, y, z);
 }
 
+// The thread's cell; 32-bit unsigned index arithmetic (a box holds fewer
+// than 2^31 cells).
 __device__ __forceinline__ bool cell_of(const Box &b, int &x, int &y, int &z, long &a) {
   a = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= b.ncell) return false;
-  x = (int)(a % b.n[0]);
-  y = (int)((a / b.n[0]) % b.n[1]);
-  z = (int)(a / b.n[0] / b.n[1]);
+  const unsigned u = (unsigned)a, n0 = (unsigned)b.n[0], n1 = (unsigned)b.n[1];
+  const unsigned row = u / n0;
+  x = (int)(u - row * n0);
+  z = (int)(row / n1);
+  y = (int)(row - (unsigned)z * n1);
   return true;
 }
 
@@ -218,21 +222,27 @@ __device__ __forceinline__ void adj_face(const Box &b, const FieldPtrs &W, const
     }
   }
   // recompute the face states (same reconstruction as the forward)
-  const long c0 = nb(b, x, y, z, D, -2), c1 = m1, c2 = a, c3 = nb(b, x, y, z, D, 1);
-  auto bavg = [&](int t, long c) -> float {
-    int cx = (int)(c % b.n[0]), cy = (int)((c / b.n[0]) % b.n[1]), cz = (int)(c / b.n[0] / b.n[1]);
-    return 0.5f * (W.f[5 + t][c] + W.f[5 + t][nb(b, cx, cy, cz, t, 1)]);
+  // stencil cells c-2..c+1 along D, with their coordinates
+  long cs[4];
+  int cc[4][3];
+  for (int s = 0; s < 4; ++s) {
+    cc[s][0] = x, cc[s][1] = y, cc[s][2] = z;
+    cc[s][D] = wrap(cc[s][D] + s - 2, b.n[D]);
+    cs[s] = cell_index(b, cc[s][0], cc[s][1], cc[s][2]);
+  }
+  // cell-centred B_t of stencil cell s: the average of its two t-faces
+  auto bavg = [&](int t, int s) -> float {
+    return 0.5f * (W.f[5 + t][cs[s]] + W.f[5 + t][nb(b, cc[s][0], cc[s][1], cc[s][2], t, 1)]);
   };
   float q[7][4];
-  const long cs[4] = {c0, c1, c2, c3};
   for (int s = 0; s < 4; ++s) {
     q[0][s] = W.f[0][cs[s]];
     q[1][s] = W.f[1 + D][cs[s]];
     q[2][s] = W.f[1 + D1][cs[s]];
     q[3][s] = W.f[1 + D2][cs[s]];
     q[4][s] = W.f[4][cs[s]];
-    q[5][s] = bavg(D1, cs[s]);
-    q[6][s] = bavg(D2, cs[s]);
+    q[5][s] = bavg(D1, s);
+    q[6][s] = bavg(D2, s);
   }
   float Lq[7], Rq[7];
   for (int k = 0; k < 7; ++k) {
@@ -301,12 +311,17 @@ template <int D>
 __device__ __forceinline__ void adj_gather(const Box &b, const FieldPtrs &W, const AdjFaceOut &fa,
                                            int x, int y, int z, float (&res)[7]) {
   constexpr int D1 = (D + 1) % 3, D2 = (D + 2) % 3;
-  // cells c-2..c+2 and faces c-1..c+2 along D
+  // cells c-2..c+2 and faces c-1..c+2 along D, with their coordinates
   long cs[5];
-  for (int s = 0; s < 5; ++s) cs[s] = nb(b, x, y, z, D, s - 2);
-  auto bavg = [&](int t, long c) -> float {
-    int cx = (int)(c % b.n[0]), cy = (int)((c / b.n[0]) % b.n[1]), cz = (int)(c / b.n[0] / b.n[1]);
-    return 0.5f * (W.f[5 + t][c] + W.f[5 + t][nb(b, cx, cy, cz, t, 1)]);
+  int cc[5][3];
+  for (int s = 0; s < 5; ++s) {
+    cc[s][0] = x, cc[s][1] = y, cc[s][2] = z;
+    cc[s][D] = wrap(cc[s][D] + s - 2, b.n[D]);
+    cs[s] = cell_index(b, cc[s][0], cc[s][1], cc[s][2]);
+  }
+  // cell-centred B_t of stencil cell s: the average of its two t-faces
+  auto bavg = [&](int t, int s) -> float {
+    return 0.5f * (W.f[5 + t][cs[s]] + W.f[5 + t][nb(b, cc[s][0], cc[s][1], cc[s][2], t, 1)]);
   };
   for (int k = 0; k < 7; ++k) {
     float xv[5];
@@ -317,8 +332,8 @@ __device__ __forceinline__ void adj_gather(const Box &b, const FieldPtrs &W, con
             : k == 2 ? W.f[1 + D1][c]
             : k == 3 ? W.f[1 + D2][c]
             : k == 4 ? W.f[4][c]
-            : k == 5 ? bavg(D1, c)
-                     : bavg(D2, c);
+            : k == 5 ? bavg(D1, s)
+                     : bavg(D2, s);
     }
     // face f of cell cs[s] is its lower face; aL(face c+1) = fa.aL[k][cs[3]]
     const float aL_c = fa.aL[k][cs[2]], aL_c1 = fa.aL[k][cs[3]], aL_c2 = fa.aL[k][cs[4]];
