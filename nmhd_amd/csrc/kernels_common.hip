@@ -45,9 +45,11 @@ __global__ void k_ghost_y(Box b, FieldPtrs W, const StepState *st) {
   const long unit = (long)nx * M * b.n[2];
   const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= unit) return;
-  const int sx = (int)(a % nx);
-  const int sy = (int)((a / nx) % M);
-  const int sz = (int)(a / nx / M);
+  // 32-bit index arithmetic: the slab holds fewer than 2^31 cells
+  const unsigned u = (unsigned)a, row = u / (unsigned)nx;
+  const int sx = (int)(u - row * (unsigned)nx);
+  const int sz = (int)(row / (unsigned)M);
+  const int sy = (int)(row - (unsigned)sz * (unsigned)M);
   int n = 0;
   float dr = 0.0f, nd, relvel = 0.0f;
   int ix = sx, dx = sx;
@@ -146,16 +148,18 @@ __global__ void __launch_bounds__(256) k_cfl_ghosts(Box b, FieldPtrs W, int slab
   float t = 3.0e38f;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long)gridDim.x * blockDim.x) {
-    int x, y, z;
+    // 32-bit index arithmetic: a box holds fewer than 2^31 cells
+    const unsigned u = (unsigned)i, row = u / (unsigned)nx;
+    const int x = (int)(u - row * (unsigned)nx);
+    int y, z;
     if (slab == 0) {
-      x = (int)(i % nx);
-      const int r = (int)((i / nx) % (2 * M));
-      z = (int)(i / nx / (2 * M));
+      const unsigned m2 = 2u * (unsigned)M;
+      z = (int)(row / m2);
+      const int r = (int)(row - (unsigned)z * m2);
       y = r < M ? r : ny - 2 * M + r;
     } else {
-      x = (int)(i % nx);
-      y = (int)((i / nx) % ny);
-      const int r = (int)(i / nx / ny);
+      const int r = (int)(row / (unsigned)ny);
+      y = (int)(row - (unsigned)r * (unsigned)ny);
       z = r < M ? r : nz - 2 * M - 1 + r;
     }
     t = fminf(t, cell_cfl(b, W, addr(b, x, y, z), x, y, z));
