@@ -103,7 +103,7 @@ __global__ void __launch_bounds__(tiled::kTileThreads)
 
 // One workgroup per work item of one tile shape.
 template <int TX, int TY>
-__global__ void __launch_bounds__(tiled::kTileThreads, 4)
+__global__ void __launch_bounds__(tiled::kTileThreads, (tiled::FacesShape<TX, TY>::kWavesPerSimd))
     k_batch_faces_items(const BatchBox *boxes, const WorkItem *items, int nitems, float *state,
                         float *flux, const StepState *st, int hsel) {
   __shared__ float lds[tiled::FacesShape<TX, TY>::kLds];
@@ -169,9 +169,8 @@ __global__ void __launch_bounds__(256) k_batch_cfl(const BatchBox *boxes, float 
   float t = goal;
   for (long a = (long)blockIdx.x * blockDim.x + threadIdx.x; a < b.ncell;
        a += (long)gridDim.x * blockDim.x) {
-    const int x = (int)(a % b.n[0]);
-    const int y = (int)((a / b.n[0]) % b.n[1]);
-    const int z = (int)(a / b.n[0] / b.n[1]);
+    int x, y, z;
+    cell_coords(b, a, x, y, z);
     t = fminf(t, cell_cfl(b, W, a, x, y, z));
   }
   const float m = tiled::wave_min64(t);

@@ -128,9 +128,8 @@ __global__ void __launch_bounds__(256) k_cfl(Box b, FieldPtrs W, float goal, Ste
   float t = goal;
   for (long a = (long)blockIdx.x * blockDim.x + threadIdx.x; a < b.ncell;
        a += (long)gridDim.x * blockDim.x) {
-    const int x = (int)(a % b.n[0]);
-    const int y = (int)((a / b.n[0]) % b.n[1]);
-    const int z = (int)(a / b.n[0] / b.n[1]);
+    int x, y, z;
+    cell_coords(b, a, x, y, z);
     t = fminf(t, cell_cfl(b, W, a, x, y, z));
   }
   cfl_block_fold(t, st);

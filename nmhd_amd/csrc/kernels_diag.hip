@@ -54,7 +54,8 @@ __global__ void __launch_bounds__(kDiagBlock) k_diag_partial(Box b, FieldPtrs W,
   const float dmin = fminf(b.d[0], fminf(b.d[1], b.d[2]));
   for (long a = (long)blockIdx.x * blockDim.x + threadIdx.x; a < b.ncell;
        a += (long)gridDim.x * blockDim.x) {
-    const int x = (int)(a % nx), y = (int)((a / nx) % ny), z = (int)(a / nx / ny);
+    int x, y, z;
+    cell_coords(b, a, x, y, z);
     const float bx = W.f[5][a], by = W.f[6][a], bz = W.f[7][a];
     // the monitor squares in the field's precision and sums in FP64
     v[0] += (double)(bx * bx);

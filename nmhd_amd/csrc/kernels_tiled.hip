@@ -16,7 +16,8 @@ namespace tiled {
 
 // Recon = b.recon (the launcher picks the instantiation)
 template <int TX, int TY, int Recon>
-__global__ void __launch_bounds__(kTileThreads, 4) k_faces(Box b, FieldPtrs W, FluxSet F,
+__global__ void __launch_bounds__(kTileThreads, (FacesShape<TX, TY>::kWavesPerSimd))
+    k_faces(Box b, FieldPtrs W, FluxSet F,
                                                         const StepState *st, int hsel, int zc,
                                                         int gx, int gy) {
   if (!st->active) return;

@@ -38,6 +38,16 @@ void host_displacement(const Box &b, double t, float *normalized, int *n, float 
 // Start a step.  full: the CFL minimum is rebuilt over every cell by a full
 // CFL pass; otherwise it starts from the stage-2 epilogue's minimum of the
 // current state (plus, with ghosts, the cells the ghost fill rewrote).
+// (x, y, z) of the cell at linear index a (x fastest), in 32-bit unsigned
+// arithmetic: a box holds fewer than 2^31 cells.
+__device__ __forceinline__ void cell_coords(const Box &b, long a, int &x, int &y, int &z) {
+  const unsigned u = (unsigned)a, n0 = (unsigned)b.n[0], n1 = (unsigned)b.n[1];
+  const unsigned row = u / n0;
+  x = (int)(u - row * n0);
+  z = (int)(row / n1);
+  y = (int)(row - (unsigned)z * n1);
+}
+
 __device__ __forceinline__ void step_begin(StepState &s, int full) {
   s.cfl_bits = full ? 0x7f800000u : s.cfl_next_bits;
   s.cfl_next_bits = 0x7f800000u;  // +inf

@@ -148,6 +148,9 @@ struct FacesShape {
   // rather than in registers (read once per plane by the slot's own thread)
   static constexpr int kGxy = kFy + 5 * (TY + 1) * TX;
   static constexpr int kLds = kGxy + LN;
+  // waves per SIMD the LDS allows (4 workgroups of 4 waves per CU when 4
+  // copies fit the 160 KiB, else 3): the kernels' occupancy target
+  static constexpr int kWavesPerSimd = 4 * kLds * 4 <= 160 * 1024 ? 4 : 3;
 };
 
 // Face fluxes of a TX x TY column tile over planes [z0, zend), and the
