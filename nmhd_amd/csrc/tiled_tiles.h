@@ -356,6 +356,14 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
     const int cb = p & 1, nb = (p + 1) & 1;
     const bool prefetch = (p + 2 <= zlast);
     const unsigned ownp = opaque(own);
+    if constexpr (TX + TY <= 64) {
+      // the wave that also evaluates this plane's boundary faces sets the
+      // workgroup's pace at the plane barrier: it issues first
+      if (__builtin_amdgcn_readfirstlane(tid >> 6) == ((p + x0 / TX + y0 / TY) & 3))
+        __builtin_amdgcn_s_setprio(1);
+      else
+        __builtin_amdgcn_s_setprio(0);
+    }
     if (prefetch) load_plane(p + 2);
     {
       const float *P0 = lds + cb * 8 * LN;
