@@ -17,6 +17,7 @@
 #include <sys/stat.h>
 #include <sys/time.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -251,6 +252,21 @@ class LocalSetup {
   }
 };
 
+// The setup text of save_state / dump (INL:1250-1259): the GlobalSetup
+// block, an empty line, a line of 16 '#', the LocalSetup block and an empty
+// line.  read_setup is load_state's reader (INL:1261-1267): the two blocks
+// around the spacer word, each a count and that many "key value" pairs in any
+// order; only the members the reference restores are taken.
+inline void write_setup(std::ostream &o, const GlobalSetup &g, const LocalSetup &l) {
+  o << g << std::endl << "################" << std::endl << l << std::endl;
+}
+inline void read_setup(std::istream &in, GlobalSetup &g, LocalSetup &l) {
+  in >> g;
+  std::string spacer;
+  in >> spacer;
+  in >> l;
+}
+
 struct PositioningSetup {  // crystal.hrb:11-15
   Triplet<Real> corner;
   int mpi_rank_y;
@@ -264,6 +280,12 @@ class MHDSolver {
   std::function<void(const PositioningSetup &, const RCo<7> &, BodyScalar &, BodyVector &,
                      BodyScalar &, FaceVector &)>
       set_initial_condition;
+  // Boundary condition on the host, called for every cell including ghosts
+  // (set_boundary_condition through boundary_condition_kernel, INL:56-66);
+  // every reference application leaves it empty.
+  std::function<void(const PositioningSetup &, const RCo<7> &, BodyScalar &, BodyVector &,
+                     BodyScalar &, FaceVector &)>
+      set_boundary_condition;
   std::function<int(MHDSolver &)> monitor_hook;      // non-zero breaks proceed()
   std::function<int(MHDSolver &)> initial_event_hook;
   std::function<int(MHDSolver &)> final_event_hook;
@@ -411,16 +433,12 @@ class MHDSolver {
     write(prefix + ".bin");
     update_displacement();
     std::ofstream ofs((prefix + "_setup.txt").c_str());
-    ofs << global_setup_ << std::endl << "################" << std::endl << local_setup_
-        << std::endl;
+    write_setup(ofs, global_setup_, local_setup_);
   }
   void load_state(const std::string &tag) {
     const std::string prefix = local_setup_.mkdir_prefix(tag);
     std::ifstream ifs((prefix + "_setup.txt").c_str());
-    ifs >> global_setup_;
-    std::string spacer;
-    ifs >> spacer;
-    ifs >> local_setup_;
+    read_setup(ifs, global_setup_, local_setup_);
     if (!read(prefix + ".bin"))
       std::cerr << "error reading file: " << prefix << ".bin" << std::endl;
     sync_time();
@@ -433,8 +451,7 @@ class MHDSolver {
     const std::string prefix = local_setup_.mkdir_prefix(tag);
     write(prefix + ".bin");
     std::ofstream ofs((prefix + "_setup.txt").c_str());
-    ofs << global_setup_ << std::endl << "################" << std::endl << local_setup_
-        << std::endl;
+    write_setup(ofs, global_setup_, local_setup_);
   }
 
   int monitor() { return monitor_hook ? monitor_hook(*this) : 0; }
@@ -449,6 +466,57 @@ class MHDSolver {
   }
   int initial_event() { return initial_event_hook ? initial_event_hook(*this) : 0; }
   int final_event() { return final_event_hook ? final_event_hook(*this) : 0; }
+
+  // communicate (INL:307-711): the ghost layers of the state from its
+  // interior at the current time -- shearing-periodic along the radial
+  // axis Y when the box is a shearing box, plain periodic otherwise, and
+  // plain periodic along Z.  proceed() does this at the start of every step.
+  void communicate() { check(nmhd_fill_ghosts(ctx_), "nmhd_fill_ghosts"); }
+
+  // boundary_condition (INL:56-66): set_boundary_condition for every cell.
+  // Without a hook (every reference application) this is a no-op and, like
+  // the reference's empty kernel, touches nothing; with one, the state makes
+  // a host round trip.
+  void boundary_condition() {
+    if (!set_boundary_condition) return;
+    sync_to_host();
+    BodyScalar rho = density(), p = pressure();
+    BodyVector v = velocity();
+    FaceVector b = magnet();
+    const PositioningSetup pos = positioning_setup();
+    for (size_t a = 0; a < geo_.ncell; ++a)
+      set_boundary_condition(pos, RCo<7>(lat_, (long)a), rho, v, p, b);
+    sync_to_device();
+  }
+
+  // add_flux_kernel_caller (INL:918-988): adds the right-hand side of the
+  // state (density, velocity, pressure, magnet) -- the X, Y, Z sweeps of
+  // face fluxes, their divergence and the constrained-transport EMF
+  // differences, with the source term's face predictor taken over the step
+  // dt -- to the eight pace arrays (the reference calls it after clear_pace,
+  // so the paces then hold the right-hand side).  Views are host arrays of
+  // the solver's size; a state other than the solver's own views
+  // (density(), velocity(), pressure(), magnet()) becomes the solver's state.
+  void add_flux_kernel_caller(Real dt, BodyScalar density, BodyVector velocity,
+                              BodyScalar pressure, FaceVector magnet, BodyScalar density_pace,
+                              BodyVector momentum_pace, BodyScalar energy_pace,
+                              FaceVector magnet_pace) {
+    const Real *in[8] = {density.ptr(),  velocity[X].ptr(), velocity[Y].ptr(),
+                         velocity[Z].ptr(), pressure.ptr(),  magnet[X].ptr(),
+                         magnet[Y].ptr(), magnet[Z].ptr()};
+    for (int f = 0; f < 8; ++f)
+      if (in[f] != &host_[f][0]) std::copy(in[f], in[f] + geo_.ncell, host_[f].begin());
+    sync_to_device();
+    std::vector<std::vector<Real> > pace(8, std::vector<Real>(geo_.ncell));
+    float *pp[8];
+    for (int f = 0; f < 8; ++f) pp[f] = &pace[f][0];
+    check(nmhd_rhs(ctx_, dt, pp), "nmhd_rhs");
+    Real *out[8] = {density_pace.ptr(),  momentum_pace[X].ptr(), momentum_pace[Y].ptr(),
+                    momentum_pace[Z].ptr(), energy_pace.ptr(),    magnet_pace[X].ptr(),
+                    magnet_pace[Y].ptr(), magnet_pace[Z].ptr()};
+    for (int f = 0; f < 8; ++f)
+      for (size_t a = 0; a < geo_.ncell; ++a) out[f][a] += pace[f][a];
+  }
 
   // relative_displacement / update_displacement (INL:288-303)
   void relative_displacement(PreciseReal t, int index, Real &normalized, int &int_displacement,

@@ -48,6 +48,8 @@ def build_cpp():
     flags = ["g++", "-std=c++14", "-O2", "-Wall"]
     _run(flags + inc + [os.path.join(ROOT, "tests", "cpp", "crystal_test.cpp"),
                         "-o", os.path.join(tbin, "crystal_test")])
+    _run(flags + inc + [os.path.join(ROOT, "tests", "cpp", "setup_format_test.cpp"),
+                        "-o", os.path.join(tbin, "setup_format_test")])
     _run(flags + inc + [os.path.join(ROOT, "tests", "cpp", "facade_test.cpp"),
                         "-o", os.path.join(tbin, "facade_test")] + link +
          ["-Wl,-rpath,$ORIGIN/../../../nmhd_amd"])

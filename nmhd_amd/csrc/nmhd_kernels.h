@@ -48,7 +48,11 @@ __device__ __forceinline__ void cell_coords(const Box &b, long a, int &x, int &y
   y = (int)(row - (unsigned)z * n1);
 }
 
+// A step enqueued after the run reached its goal is inactive and leaves the
+// record alone: cfl_next_bits then still holds the last active step's
+// minimum, which the first step of the next run starts from.
 __device__ __forceinline__ void step_begin(StepState &s, int full) {
+  if (s.finished) return;
   s.cfl_bits = full ? 0x7f800000u : s.cfl_next_bits;
   s.cfl_next_bits = 0x7f800000u;  // +inf
 }

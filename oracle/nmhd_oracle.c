@@ -676,6 +676,14 @@ void or_rhs(or_ctx *c, float h, float *pace[8]) {
     memcpy(pace[f], c->pace[f], sizeof(float) * c->ncell);
 }
 
+/* calc_flux_kernel<EX> alone (INL:760-816) for sweep d of the current state:
+ * the 7 face-flux arrays of the lower d-faces (mass, momentum along X, Y, Z,
+ * energy, B along d+1, B along d+2). */
+void or_face_flux(or_ctx *c, int d, float h, float *out[7]) {
+  calc_flux(c, d, h, c->w);
+  for (int f = 0; f < 7; ++f) memcpy(out[f], c->flux[f], sizeof(float) * c->ncell);
+}
+
 /* MHDSolver::proceed loop body, INL:1094-1201 (INLRB:858-990) */
 float or_step(or_ctx *c, float goal_time, int *goal_reached) {
   *goal_reached = 0;

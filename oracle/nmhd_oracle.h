@@ -78,6 +78,10 @@ void or_cfl_field(or_ctx *c, float goal_time, float *out);
  * of the current state with predictor step h, copied to pace[8]
  * (rho, mx, my, mz, E, Bx, By, Bz). */
 void or_rhs(or_ctx *c, float h, float *pace[8]);
+/* calc_flux_kernel<EX> (INL:760-816) of sweep d (0 = X, 1 = Y, 2 = Z) on the
+ * current state with predictor step h: out[7] = mass, momentum X, Y, Z,
+ * energy, B_{d+1}, B_{d+2} fluxes through the lower d-face of every cell. */
+void or_face_flux(or_ctx *c, int d, float h, float *out[7]);
 /* One iteration of MHDSolver::proceed's loop (INL:1094-1201) without the
  * monitor/wallclock hooks.  Returns the dt used; *goal_reached set when the
  * step was clipped to goal_time. */

@@ -62,6 +62,7 @@ def lib():
         L.or_cfl_dt.argtypes = [ctypes.c_void_p, ctypes.c_float]
         L.or_cfl_field.argtypes = [ctypes.c_void_p, ctypes.c_float, _FP]
         L.or_rhs.argtypes = [ctypes.c_void_p, ctypes.c_float, _FPP]
+        L.or_face_flux.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_float, _FPP]
         L.or_step.restype = ctypes.c_float
         L.or_step.argtypes = [ctypes.c_void_p, ctypes.c_float,
                               ctypes.POINTER(ctypes.c_int)]
@@ -209,6 +210,13 @@ class Oracle:
     def rhs(self, h):
         out = [np.zeros(self.shape, np.float32) for _ in range(8)]
         lib().or_rhs(self.h, h, _ptrs([o.reshape(-1) for o in out]))
+        return np.stack(out)
+
+    def face_flux(self, d, h):
+        """(7,) + shape fluxes through the lower d-faces of the current
+        state: mass, momentum x, y, z, energy, B_{d+1}, B_{d+2}."""
+        out = [np.zeros(self.shape, np.float32) for _ in range(7)]
+        lib().or_face_flux(self.h, d, h, _ptrs([o.reshape(-1) for o in out]))
         return np.stack(out)
 
     def step(self, goal=1e30):

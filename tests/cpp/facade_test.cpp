@@ -129,6 +129,92 @@ int main(int argc, char **argv) {
     EXPECT(reached);
     EXPECT(std::fabs(g.current_time() - 0.05) < 1e-7);
   }
+  // 6. the internal step calls: add_flux_kernel_caller adds the right-hand
+  //    side to the paces (twice = doubled; a uniform state has none),
+  //    communicate() fills the shearing box's radial ghost rows from the
+  //    interior (at t = 0 the lower rows are the top interior rows shifted
+  //    by 0 cells, with v_az - relvel), boundary_condition() without a hook
+  //    leaves the state alone
+  {
+    LocalSetup lh(0, dir + "/h");
+    MHDSolver h(cfg, lh);
+    ot_ic(h);
+    h.initialize_state();
+    const size_t n = 16 * 16 * 16;
+    std::vector<std::vector<Real> > pace(8, std::vector<Real>(n, Real(0)));
+    BodyScalar dp(&pace[0][0]), ep(&pace[4][0]);
+    BodyScalar m0(&pace[1][0]), m1(&pace[2][0]), m2(&pace[3][0]);
+    BodyVector mp(m0, m1, m2);
+    FaceVector bp(&pace[5][0], &pace[6][0], &pace[7][0]);
+    h.add_flux_kernel_caller(Real(0.001), h.density(), h.velocity(), h.pressure(), h.magnet(),
+                             dp, mp, ep, bp);
+    std::vector<std::vector<Real> > once = pace;
+    h.add_flux_kernel_caller(Real(0.001), h.density(), h.velocity(), h.pressure(), h.magnet(),
+                             dp, mp, ep, bp);
+    double mx = 0;
+    bool doubled = true;
+    for (int f = 0; f < 8; ++f)
+      for (size_t a = 0; a < n; ++a) {
+        mx = std::fmax(mx, std::fabs(once[f][a]));
+        doubled = doubled && pace[f][a] == once[f][a] + once[f][a];
+      }
+    EXPECT(mx > 0.1 && doubled);
+    std::vector<std::vector<Real> > uni(8, std::vector<Real>(n, Real(0.5)));
+    std::vector<std::vector<Real> > zp(8, std::vector<Real>(n, Real(0)));
+    h.add_flux_kernel_caller(
+        Real(0.001), BodyScalar(&uni[0][0]),
+        BodyVector(BodyScalar(&uni[1][0]), BodyScalar(&uni[2][0]), BodyScalar(&uni[3][0])),
+        BodyScalar(&uni[4][0]), FaceVector(&uni[5][0], &uni[6][0], &uni[7][0]),
+        BodyScalar(&zp[0][0]),
+        BodyVector(BodyScalar(&zp[1][0]), BodyScalar(&zp[2][0]), BodyScalar(&zp[3][0])),
+        BodyScalar(&zp[4][0]), FaceVector(&zp[5][0], &zp[6][0], &zp[7][0]));
+    bool zero = true;
+    for (int f = 0; f < 8; ++f)
+      for (size_t a = 0; a < n; ++a) zero = zero && zp[f][a] == 0;
+    EXPECT(zero);
+    h.sync_to_host();
+    EXPECT(h.host_field(0)[0] == Real(0.5));  // the uniform state became the solver's
+  }
+  {
+    SimulationConfig sb;
+    sb.global_size[0] = 16;
+    sb.global_size[1] = 16;
+    sb.global_size[2] = 8;
+    sb.margin[1] = 4;
+    sb.global_corner[0] = sb.global_corner[1] = Real(-0.5);
+    sb.cfl_factor = 0.4f;
+    sb.source = NMHD_SOURCE_SHEARING_BOX;
+    sb.shearing_boundary = true;
+    LocalSetup ls(0, dir + "/s");
+    MHDSolver s(sb, ls);
+    s.set_initial_condition = [](const PositioningSetup &, const RCo<7> &o, BodyScalar &rho,
+                                 BodyVector &v, BodyScalar &p, FaceVector &B) {
+      rho[o] = Real(1 + 0.01 * o[X] + 0.001 * o[Y] + 0.0001 * o[Z]);
+      p[o] = 1;
+      v[X][o] = Real(0.01 * o[Y]);
+      v[Y][o] = 0;
+      v[Z][o] = 0;
+      B[X][o - HX] = 0;
+      B[Y][o - HY] = 0;
+      B[Z][o - HZ] = Real(0.07);
+    };
+    s.initialize_state();
+    s.boundary_condition();  // no hook: nothing happens
+    s.communicate();
+    s.sync_to_host();
+    const int nx = 16, ny = 24, M = 4;
+    const Real relvel = Real(-1.5f * 1.0f * 1.0f);
+    bool ok = true;
+    for (int z = 0; z < 8; ++z)
+      for (int j = 0; j < M; ++j)
+        for (int x = 0; x < nx; ++x) {
+          const size_t dst = ((size_t)z * ny + j) * nx + x;
+          const size_t src = ((size_t)z * ny + (ny - 2 * M + j)) * nx + x;
+          ok = ok && s.host_field(0)[dst] == s.host_field(0)[src];
+          ok = ok && s.host_field(1)[dst] == s.host_field(1)[src] - relvel;
+        }
+    EXPECT(ok);
+  }
   std::printf("facade_test: %d failures\n", failures);
   return failures ? 1 : 0;
 }

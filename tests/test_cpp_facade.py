@@ -25,6 +25,15 @@ def test_crystal_addressing_laws():
     assert r.returncode == 0, r.stdout + r.stderr
 
 
+def test_setup_text_known_answer():
+    """save_state's setup text, byte for byte against a file written by hand
+    from the reference's encoder rules, and load_state's reader on a
+    reference-format file with %-escaped strings (tests/golden/setup_*.txt)."""
+    r = run("setup_format_test", os.path.join(ROOT, "tests", "golden"))
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
 def test_facade_header_compiles_standalone(tmp_path):
     src = tmp_path / "inc.cpp"
     src.write_text('#include "nmhd/mhd_solver.h"\nint main() { return 0; }\n')

@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 import pyoracle as po
-from mhd_checks import bitwise_equal, div_b, group_rel_err
+from mhd_checks import bitwise_equal, div_b, group_rel_err, perturb_ulps
 from nmhd_amd import capi, problems as P
 
 pytestmark = pytest.mark.gpu
@@ -200,3 +200,73 @@ def test_batch_thin_boxes_equal_single_boxes(gpu_ready):
     B = make_batch(boxes)
     B.run(0.05, 20, use_graph=True)
     assert_batch_equals(B, ref)
+
+
+def test_c4_workload_autotuned_graph_steps(gpu_ready, tmp_path):
+    """The multi-box workload itself: 64 boxes of edges from {16, ..., 96}
+    (seed 1004; Orszag-Tang and blast boxes), autotuned, then 5 steps
+    replayed from the captured graph.  Every box equals a single-box tiled
+    context bit for bit (state and time), and two Orszag-Tang and two blast
+    boxes stay within three times the oracle's own one-ulp sensitivity of
+    the oracle (both states are mirror symmetric, so CT upwind sums are
+    exactly zero on whole rows of edges and their choice is a round-off
+    tie)."""
+    announce("C4: 64 boxes, autotune, 5 graph steps vs single contexts and oracle")
+    boxes = P.batch_boxes(seed=1004, nbox=64)
+    B = make_batch(boxes)
+    B.autotune(str(tmp_path / "tune.txt"))
+    print("launch (faces zchunk, update zchunk, faces wg/CU, update wg/CU):", B.get_launch(),
+          flush=True)
+    k, reached = B.run(1e30, 5, use_graph=True)
+    assert k == 5 and not reached
+    ref = single_runs(boxes, 1e30, 5)
+    assert_batch_equals(B, ref)
+    checked = {0: [], 1: []}
+    for i, (cfg, W) in enumerate(boxes):
+        kind = i % 2
+        if len(checked[kind]) == 2:
+            continue
+        o = po.Oracle(cfg, W)
+        o2 = po.Oracle(cfg, perturb_ulps(W))
+        o.run(1e30, 5)
+        o2.run(1e30, 5)
+        want = o.get_state()
+        env = group_rel_err(o2.get_state(), want)
+        err = group_rel_err(B.get_state(i), want)
+        print("box %d %s %s: err %s, oracle 1-ulp envelope %s"
+              % (i, "OT" if kind == 0 else "blast", cfg["global_size"],
+                 np.array2string(err, precision=2), np.array2string(env, precision=2)))
+        assert np.all(err <= 3.0 * env + 1e-6), (i, err, env)
+        assert B.get_time(i)[0] == pytest.approx(o.time, rel=1e-6)
+        checked[kind].append(i)
+    assert len(checked[0]) == len(checked[1]) == 2
+
+
+def test_batch_second_run_after_goal(gpu_ready):
+    """Boxes that reached the goal keep the CFL minimum of their last active
+    step through the trailing inactive steps, so a second run to a later goal
+    takes the same steps as single contexts whose second run recomputes the
+    CFL over the whole box."""
+    announce("batch run(g1) then run(g2) vs single contexts")
+    boxes = small_boxes(seed=8)
+    probe = make_batch(boxes)
+    probe.run(1e30, 1, use_graph=False)
+    dts = sorted(probe.status(i)[2] for i in range(probe.nbox))
+    g1 = float(np.float32(3.5 * dts[len(dts) // 2]))
+    g2 = float(np.float32(2.0 * g1))
+    ref = []
+    for cfg, W in boxes:
+        g = capi.Context(cfg, path=capi.PATH_TILED)
+        g.set_state(W)
+        g.run(g1, 40)
+        g.set_path(capi.PATH_TILED)  # the second run starts from a full CFL pass
+        k, _ = g.run(g2, 40)
+        ref.append((g.get_state(), g.get_time(), k))
+        g.close()
+    B = make_batch(boxes)
+    B.run(g1, 40, use_graph=True)
+    assert all(B.status(i)[1] for i in range(B.nbox))
+    B.run(g2, 40, use_graph=True)
+    assert_batch_equals(B, ref)
+    for i in range(B.nbox):
+        assert B.status(i)[0] == ref[i][2], i

@@ -404,3 +404,103 @@ def test_two_contexts_interleaved_equal_alone(gpu_ready):
         assert k == 6
         assert bitwise_equal(g.get_state(), ref)
         g.close()
+
+
+def test_tiled_second_run_after_goal(gpu_ready):
+    """A run that reaches its goal part-way through an enqueued chunk of
+    steps must leave the CFL minimum of its last active step for the next
+    run (the reference's drivers call proceed() again and again,
+    kepler_disk_demo/main.curb:97).  run(g1) then run(g2) on the tiled path
+    (incremental CFL) takes the oracle's step count, equals bit for bit a
+    context whose second run recomputes the CFL over the whole box, and
+    stays within the fast-path tolerance of the oracle."""
+    announce("tiled run(g1) then run(g2) vs full-CFL restart and oracle")
+    cfg, W = case("sb16")
+    o = po.Oracle(cfg, W)
+    dt0 = o.cfl_dt(1e30)
+    g1 = float(np.float32(5.5 * dt0))
+    g2 = float(np.float32(g1 + 4.3 * dt0))
+    a = capi.Context(cfg, path=capi.PATH_TILED)
+    b = capi.Context(cfg, path=capi.PATH_TILED)
+    a.set_state(W)
+    b.set_state(W)
+    k_o1, r_o1 = o.run(g1, 1000)
+    k_a1, r_a1 = a.run(g1, 1000)  # one 32-step chunk: the goal falls inside it
+    k_b1, _ = b.run(g1, 1000)
+    assert r_o1 and r_a1 and k_a1 == k_b1 == k_o1
+    assert 4 <= k_o1 < 32  # inactive steps follow the goal in the same chunk
+    b.set_path(capi.PATH_TILED)  # forget the carried CFL minimum
+    k_o2, r_o2 = o.run(g2, 1000)
+    k_a2, r_a2 = a.run(g2, 1000)
+    k_b2, _ = b.run(g2, 1000)
+    print("second run: steps oracle %d, carried CFL %d, full CFL %d" % (k_o2, k_a2, k_b2))
+    assert r_o2 and r_a2 and k_a2 == k_b2 == k_o2 >= 4
+    assert a.get_time() == b.get_time()
+    assert bitwise_equal(a.get_state(), b.get_state())
+    assert a.get_time()[0] == pytest.approx(o.time, abs=1e-7)
+    got, want = a.get_state(), o.get_state()
+    assert group_frac_within(got, want, 1e-4) >= 0.999
+    assert group_rel_err(got, want).max() < 1e-3
+
+
+@pytest.mark.parametrize("path", [capi.PATH_STRICT, capi.PATH_TILED])
+def test_radial_ghost_rows_set_dt(gpu_ready, path):
+    """Interior at rest (rho = p = 1, B = 0) in the shearing box: the lower
+    radial ghost rows carry v_az = q Omega L after the shear fill and set the
+    CFL time (mhd_solver.inl:1111-1114).  The device's dt sequence follows
+    the oracle's over three steps: bit for bit on the strict path, and on
+    the tiled path, whose later steps take the minimum from the stage-2
+    epilogue plus a pass over the refilled ghost rows, to rounding."""
+    announce("ghost rows set dt, path %d" % path)
+    cfg = P.config_c2((16, 16, 8))
+    W = P.empty_state(cfg)
+    W[0] = 1.0
+    W[4] = 1.0
+    o, g = both(cfg, W, path=path)
+    want0 = np.float32(cfg["cfl"]) * np.float32(cfg["d"][0]) / np.float32(1.5 + np.sqrt(
+        np.float32(cfg["gamma"])))
+    for k in range(3):
+        dto, _ = o.step()
+        dtg, _ = g.step()
+        if k == 0:
+            assert np.float32(dto) == want0
+        if path == capi.PATH_STRICT:
+            assert dtg == dto, k
+        else:
+            assert abs(dtg - dto) <= 1e-6 * dto, (k, dtg, dto)
+
+
+def interior_div_b(W, cfg):
+    """max |div B| dx / max|B| over the interior rows of a box with radial
+    ghost rows (the faces of the outermost interior rows included)."""
+    m = cfg["m"][1]
+    ny = cfg["n"][1]
+    dv = div_b(W, cfg["d"])[:, m:ny - m, :]
+    return np.abs(dv).max() * cfg["d"][0] / np.abs(W[5:8]).max()
+
+
+def test_div_b_200_steps_tied_to_oracle(gpu_ready):
+    """Constrained transport over the shearing box's 200 steps: the device's
+    max |div B| dx / |B| stays within 1.5x of what the oracle itself
+    accumulates in the same 200 steps, on the strict and the tiled path.
+    CT keeps div B at zero in exact arithmetic; in FP32 every face update
+    rounds, and the oracle reaches 2.4e-6 after 200 steps of this 32^3 box
+    (2.47e-6 and 2.48e-6 from inputs moved by one ulp: the bound is the
+    reference's own round-off walk, not a constant)."""
+    announce("div B after 200 shearing-box steps vs oracle")
+    cfg = P.config_c2((32, 32, 32))
+    W = P.shearing_box_mri(cfg, seed=1002)
+    o = po.Oracle(cfg, W)
+    k, _ = o.run(1e30, 200)
+    assert k == 200
+    ref = interior_div_b(o.get_state(), cfg)
+    for path in (capi.PATH_STRICT, capi.PATH_TILED):
+        g = capi.Context(cfg, path=path)
+        g.set_state(W)
+        k, _ = g.run(1e30, 200)
+        assert k == 200
+        got = interior_div_b(g.get_state(), cfg)
+        print("div B dx/|B| after 200 steps: oracle %.3g, path %d %.3g" % (ref, path, got))
+        assert ref < 5e-6
+        assert got <= 1.5 * ref
+        g.close()

@@ -312,3 +312,70 @@ def test_goal_clipping_ends_on_goal():
     o = po.Oracle(cfg, P.orszag_tang(cfg))
     k, reached = o.run(0.05, 1000)
     assert reached and abs(o.time - 0.05) < 1e-7 and o.generation == k
+
+
+# --------------------------------- shearing-box quirks as closed forms
+
+def test_cfl_minimum_sits_in_radial_ghost_rows():
+    """The CFL minimum runs over the whole array, radial ghost rows included
+    (mhd_solver.inl:1111-1114).  With the interior at rest (rho = p = 1,
+    B = 0) the shear fill gives the lower ghost rows v_az = 0 - relvel =
+    q Omega L_rad, so dt = C dx_az / (q Omega L_rad + sqrt(gamma)) there
+    (|v + c|, mhd_solver.inl:715-720), while the interior alone would give
+    C min(d) / sqrt(gamma)."""
+    cfg = sb_config(16, 16, 8)
+    W = P.empty_state(cfg)
+    W[0] = 1.0
+    W[4] = 1.0
+    o = po.Oracle(cfg, W)
+    o.fill_ghosts()
+    f32 = np.float32
+    relvel = f32(-f32(cfg["qshear"])) * f32(cfg["omega"]) * f32(cfg["local_ext"][1])
+    c = np.sqrt(f32(cfg["gamma"]) * f32(1.0) / f32(1.0))
+    v_lower = f32(0.0) - relvel
+    assert v_lower == f32(1.5)
+    want = f32(cfg["cfl"]) * f32(cfg["d"][0]) / np.abs(v_lower + c)
+    dt = o.cfl_dt(1e30)
+    assert f32(dt) == want, (dt, want)
+    # the minimum is attained in the lower ghost rows only
+    field = o.cfl_field(1e30)
+    rows = np.unique(np.nonzero(field == f32(dt))[1])
+    M = cfg["m"][1]
+    assert rows.size and np.all(rows < M), rows
+    # the interior alone gives a different (larger) dt
+    interior = field[:, M:cfg["n"][1] - M, :].min()
+    assert interior == pytest.approx(cfg["cfl"] * min(cfg["d"]) / math.sqrt(cfg["gamma"]),
+                                     rel=1e-6)
+    assert interior > 1.5 * dt
+
+
+def test_tidal_predictor_sits_at_cell_centre_above_radial_face():
+    """The face predictor v_rad += h/2 a_rad evaluates the tidal term
+    2 q Omega^2 Y at the centre of the face's own cell, half a cell above the
+    face: `Bit & 2 == 0` parses as `Bit & (2 == 0)`, so every staggered
+    position gets +dY/2 (crystal.h:86-104; mhd_solver.inl:793-797; the face
+    overload of external_acceleration, kepler_disk_demo/mhd_solver.curb:
+    64-81), with Y measured from -kExtentY/2 over the whole array.  With
+    v = 0, B = 0 and uniform rho, p both sides of the radial face of row j
+    carry v_rad = h/2 * 2 q Omega^2 Y_c(j), and the HLLD flux of two equal
+    states is the physical mass flux rho v_rad."""
+    cfg = sb_config(16, 16, 4)
+    rho = 1.3
+    W = P.empty_state(cfg)
+    W[0] = rho
+    W[4] = 0.9
+    o = po.Oracle(cfg, W)
+    h = 0.01
+    F = o.face_flux(1, h)[0].astype(np.float64)
+    ny, dy, ext = cfg["n"][1], cfg["d"][1], cfg["ext"][1]
+    q, om = cfg["qshear"], cfg["omega"]
+    y_centre = -ext / 2 + dy * (np.arange(ny) + 0.5)
+    want = rho * 0.5 * h * 2.0 * q * om * om * y_centre
+    scale = np.abs(want).max()
+    assert np.abs(F - want[None, :, None]).max() <= 1e-5 * scale
+    # the face's own position (half a cell lower) is not what is evaluated:
+    # it would move every flux by rho h q Omega^2 dY / 2 (4% of the largest)
+    want_face = rho * 0.5 * h * 2.0 * q * om * om * (y_centre - dy / 2)
+    assert np.abs(F - want_face[None, :, None]).min() > 0.02 * scale
+    # azimuthal faces: a_az = -2 Omega v_rad = 0 at rest, no mass flux
+    assert np.abs(o.face_flux(0, h)[0]).max() <= 1e-12
