@@ -15,8 +15,10 @@ The JSON line also carries
                   against the 8 TB/s HBM peak;
   step_roofline -- the whole step: 160 algorithmic bytes per zone-update;
   cpu_baseline -- the oracle (CPU restatement of the reference step) on the
-                  host cores of the machine that runs the bench, a 2-step
-                  sample of the same box;
+                  host cores of the machine that runs the bench
+                  (OMP_NUM_THREADS when set, else every core of the process's
+                  affinity mask), a 10-step sample of the same box, with a
+                  one-thread figure beside it;
   batched      -- 64 periodic boxes of uneven sizes (edges from
                   {16,24,32,48,64,96}, seed 1004; Orszag-Tang and blast
                   states) advanced 100 steps with per-box dt by the batched
@@ -78,7 +80,7 @@ def parse():
                          "(per-stage GB/s; no CPU baseline, batched or adjoint legs)")
     ap.add_argument("--path", choices=("auto", "unfused", "tiled", "strict"), default="auto")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
     ap.add_argument("--profile-steps", type=int, default=10)
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batch-boxes", type=int, default=64)
@@ -93,10 +95,24 @@ def parse():
     return ap.parse_args()
 
 
+def host_threads():
+    """Threads for the CPU baseline: OMP_NUM_THREADS when set (the share of
+    the host this process is given), else every core this process may run
+    on.  Returns (threads, a record of what the host reports)."""
+    omp = os.environ.get("OMP_NUM_THREADS")
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = nproc
+    threads = int(omp) if omp and omp.strip().isdigit() and int(omp) > 0 else affinity
+    return threads, {"nproc": nproc, "affinity_cpus": affinity, "OMP_NUM_THREADS": omp}
+
+
 def cpu_baseline(cfg, W, steps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
-    threads = min(16, os.cpu_count() or 1)
+    threads, host = host_threads()
     orc = pyoracle.Oracle(cfg, W, threads=threads)
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -115,6 +131,7 @@ def cpu_baseline(cfg, W, steps):
         "value": P.interior_cells(cfg) * steps / el,
         "unit": "zone-updates/s",
         "cores": threads,
+        "host": host,
         "kind": "port",
         "sample": "%d steps of the same %dx%dx%d shearing box on the oracle "
                   "(C restatement of the reference step, OpenMP, %d threads)"
@@ -203,21 +220,53 @@ def adjoint_leg(a, torch):
     Wc = torch.tensor(W, dtype=torch.float32, device="cuda")
     with torch.no_grad():
         dts = A.record_dts(Wc, S, a.adjoint_steps)
-    # warm-up of both passes
-    W0 = Wc.clone().requires_grad_(True)
-    WN, _ = A.rollout(W0, S, dts=dts, checkpoint="uniform", every=8)
-    (0.5 * (WN[5:8] ** 2).sum()).backward()
+    reps = 5
+
+    def passes(lead_cycles=0):
+        """One forward (rollout + loss) and one backward.  Returns the host
+        wall times of both (each until the device finished it, when no lead
+        is given), the HIP-event times of both on torch's stream, and the
+        host time spent enqueueing the pair.  With lead_cycles a spin kernel
+        is enqueued first: the device is then busy while the host enqueues
+        both passes, so the events time the device's own work."""
+        W0 = Wc.clone().requires_grad_(True)
+        torch.cuda.synchronize()
+        if lead_cycles:
+            torch.cuda._sleep(lead_cycles)
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+        h0 = time.perf_counter()
+        ev[0].record()
+        WN, _ = A.rollout(W0, S, dts=dts, checkpoint="uniform", every=8)
+        loss = 0.5 * (WN[5:8].double() ** 2).sum()
+        ev[1].record()
+        if not lead_cycles:
+            torch.cuda.synchronize()
+        h1 = time.perf_counter()
+        loss.backward()
+        ev[2].record()
+        h2 = time.perf_counter()
+        torch.cuda.synchronize()
+        h3 = time.perf_counter()
+        return {"wall_fwd": h1 - h0, "wall_bwd": h3 - h1, "enqueue": h2 - h0,
+                "gpu_fwd": ev[0].elapsed_time(ev[1]) * 1e-3,
+                "gpu_bwd": ev[1].elapsed_time(ev[2]) * 1e-3, "grad": W0.grad}
+
+    passes()  # warm-up of both passes
+    wall = [passes() for _ in range(reps)]
+    grad = wall[-1]["grad"].double()
+    # spin-kernel rate, to size the lead: a spin of 1e7 cycles timed by events
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    torch.cuda._sleep(10 ** 7)
+    e1.record()
     torch.cuda.synchronize()
-    W0 = Wc.clone().requires_grad_(True)
-    t0 = time.perf_counter()
-    WN, _ = A.rollout(W0, S, dts=dts, checkpoint="uniform", every=8)
-    loss = 0.5 * (WN[5:8].double() ** 2).sum()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    loss.backward()
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    grad = W0.grad.double()
+    cycles_per_s = 1e7 / max(e0.elapsed_time(e1) * 1e-3, 1e-9)
+    lead_s = min(1.0, 3.0 * max(w["wall_fwd"] + w["wall_bwd"] for w in wall))
+    dev = [passes(int(lead_s * cycles_per_s)) for _ in range(reps)]
+    med = lambda rows, k: float(np.median([r[k] for r in rows]))
+    t_fwd, t_bwd = med(wall, "wall_fwd"), med(wall, "wall_bwd")
+    g_fwd, g_bwd = med(dev, "gpu_fwd"), med(dev, "gpu_bwd")
+    host_ahead = all(r["enqueue"] < lead_s for r in dev)
 
     def run_loss(Wi):
         X = Wi
@@ -263,10 +312,19 @@ def adjoint_leg(a, torch):
     rel_rms = np.abs(fds - ads) / np.sqrt((ads ** 2).mean())
     zones = P.interior_cells(cfg) * a.adjoint_steps
     return {
-        "forward_ms": (t1 - t0) * 1e3, "backward_ms": (t2 - t1) * 1e3,
-        "backward_over_forward": (t2 - t1) / (t1 - t0),
-        "forward_zone_updates_per_s": zones / (t1 - t0),
-        "adjoint_zone_updates_per_s": zones / (t2 - t1),
+        "gpu_forward_ms": g_fwd * 1e3, "gpu_backward_ms": g_bwd * 1e3,
+        "gpu_backward_over_forward": g_bwd / g_fwd,
+        "gpu_timing": "median of %d; HIP events on torch's stream, the device kept busy by a "
+                      "%.0f ms spin kernel while the host enqueued both passes (host ahead of "
+                      "the device: %s)" % (reps, lead_s * 1e3, host_ahead),
+        "wall_forward_ms": t_fwd * 1e3, "wall_backward_ms": t_bwd * 1e3,
+        "wall_backward_over_forward": t_bwd / t_fwd,
+        "wall_forward_ms_range": [min(r["wall_fwd"] for r in wall) * 1e3,
+                                  max(r["wall_fwd"] for r in wall) * 1e3],
+        "wall_timing": "median of %d; host clock until the device finished each pass "
+                       "(includes the host's launch overhead)" % reps,
+        "forward_zone_updates_per_s": zones / g_fwd,
+        "adjoint_zone_updates_per_s": zones / g_bwd,
         "fd_directions": a.fd_directions, "fd_eps": eps,
         "fd_rel_err_median": float(np.median(rel)), "fd_rel_err_max": float(np.max(rel)),
         "fd_err_over_rms_derivative_max": float(np.max(rel_rms)),

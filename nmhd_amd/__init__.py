@@ -9,4 +9,4 @@ C ABI, include/nmhd.h), with the reference's solver interface on top.
 """
 from . import problems  # noqa: F401
 
-__all__ = ["problems", "capi", "solver"]
+__all__ = ["problems", "capi", "adjoint", "postprocess"]
