@@ -292,6 +292,8 @@ typedef struct nmhd_adj nmhd_adj;
 
 int nmhd_adj_create(const nmhd_desc *desc, nmhd_adj **out);
 int nmhd_adj_destroy(nmhd_adj *adj);
+/* Message of the last failure on adj (or, for adj == NULL, of the last
+ * failed nmhd_adj_create or nmhd_adj_forward_noalloc). */
 const char *nmhd_adj_last_error(const nmhd_adj *adj);
 int nmhd_adj_get_geometry(const nmhd_adj *adj, nmhd_geometry *geo);
 /* Launch later work on `stream`, a hipStream_t such as
@@ -321,6 +323,25 @@ int nmhd_adj_forward_stages(nmhd_adj *adj, const float *const w_in[8], float dt,
 int nmhd_adj_reverse_stages(nmhd_adj *adj, const float *const w_in[8],
                             const float *const w_stage[8], const float *const w_out[8],
                             float dt, const float *const lam_out[8], float *const lam_in[8]);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (work under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): w_out = S(w_in; dt) as nmhd_adj_forward computes it,
+ * with the same kernels and bit-identical results, but with no nmhd_adj
+ * object and no allocation.  The step's scratch (step record, midpoint
+ * stage state, the face-to-update hand-off) is a fixed device array of the
+ * library, one per GPU, held until the process ends (76 MiB): boxes of at
+ * most NMHD_NOALLOC_MAX_CELLS cells; larger ones are rejected (NMHD_EINVAL).
+ * Periodic boxes only, as nmhd_adj_create.  Work is enqueued on `stream` (a
+ * hipStream_t; NULL = HIP's null stream) and the call returns at once.
+ * Every call on one GPU shares that GPU's scratch: calls that overlap on
+ * one GPU (on different streams, or from threads without ordering) race;
+ * calls on one stream are ordered by it.  The first call on a device looks
+ * the scratch up (hipGetSymbolAddress): make it outside stream capture.  A
+ * failure's message is nmhd_adj_last_error(NULL). */
+#define NMHD_NOALLOC_MAX_CELLS (1 << 20)
+int nmhd_adj_forward_noalloc(const nmhd_desc *desc, const float *const w_in[8], float dt,
+                             float *const w_out[8], void *stream);
 
 #ifdef __cplusplus
 }

@@ -55,6 +55,8 @@ def _sig(L):
         "nmhd_adj_reverse": [V, _FPA, ctypes.c_float, _FPA, _FPA],
         "nmhd_adj_forward_stages": [V, _FPA, ctypes.c_float, _OPA, _FPA],
         "nmhd_adj_reverse_stages": [V, _FPA, _OPA, _FPA, ctypes.c_float, _FPA, _FPA],
+        "nmhd_adj_forward_noalloc": [ctypes.POINTER(capi.NmhdDesc), _FPA, ctypes.c_float, _FPA,
+                                     V],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
@@ -169,6 +171,21 @@ class Stepper:
             self.h, _ptrs(W), _ptrs(Ws) if Ws is not None else None, _ptrs(Wout), float(dt),
             _ptrs(lam_out), _ptrs(lam_in)), self.h)
         return lam_in
+
+
+def forward_noalloc(cfg, W, dt, out):
+    """out = S(W; dt) through nmhd_adj_forward_noalloc on torch's current
+    stream: the library allocates nothing (its scratch is a fixed device
+    array, boxes of at most NMHD_NOALLOC_MAX_CELLS cells).  W and out are
+    caller-owned (8, nz, ny, nx) float32 CUDA tensors."""
+    L = _lib()
+    desc = capi.desc_from_config(cfg)
+    if tuple(W.shape) != tuple(out.shape):
+        raise ValueError("W and out differ in shape")
+    s = torch.cuda.current_stream().cuda_stream
+    Stepper._check(L.nmhd_adj_forward_noalloc(ctypes.byref(desc), _ptrs(W), float(dt), _ptrs(out),
+                                              ctypes.c_void_p(s)), None)
+    return out
 
 
 # ------------------------------------------------------------ checkpointing

@@ -17,7 +17,7 @@ LIB = os.path.join(HERE, "libnmhd.so")
 
 SOURCES = ["capi.hip", "capi_batch.hip", "capi_adjoint.hip", "kernels_common.hip",
            "kernels_unfused.hip", "kernels_strict.hip", "kernels_tiled.hip", "kernels_batch.hip",
-           "kernels_adjoint.hip", "kernels_diag.hip"]
+           "kernels_adjoint.hip", "kernels_diag.hip", "capi_noalloc.hip"]
 # kernels_strict.hip reproduces the reference's rounding: no FMA contraction.
 FILE_FLAGS = {"kernels_strict.hip": ["-ffp-contract=off"]}
 

@@ -33,6 +33,7 @@ EXPORTS = (
     "nmhd_adj_create", "nmhd_adj_destroy", "nmhd_adj_last_error", "nmhd_adj_get_geometry",
     "nmhd_adj_set_stream", "nmhd_adj_synchronize", "nmhd_adj_cfl_dt", "nmhd_adj_forward",
     "nmhd_adj_reverse", "nmhd_adj_forward_stages", "nmhd_adj_reverse_stages",
+    "nmhd_adj_forward_noalloc",
 )
 TILE_64x4, TILE_32x8, TILE_16x16 = 0, 1, 2
 

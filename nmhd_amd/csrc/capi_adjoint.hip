@@ -15,6 +15,25 @@
 
 using namespace nmhd;
 
+namespace nmhd {
+
+void enqueue_frozen_step(hipStream_t s, const Box &b, int integrator, FieldPtrs Win,
+                         FieldPtrs Wstar, FieldPtrs Wout, const FluxSet &F, StepState *st,
+                         float dt) {
+  launch_adj_set_dt(s, st, dt);
+  if (integrator == NMHD_EULER1) {
+    launch_faces(s, b, Win, F, st, H_FULL);
+    launch_update_tiled(s, b, Win, F, Win, Wout, st, H_FULL, false, false);
+    return;
+  }
+  launch_faces(s, b, Win, F, st, H_HALF);
+  launch_update_tiled(s, b, Win, F, Win, Wstar, st, H_HALF, false, false);
+  launch_faces(s, b, Wstar, F, st, H_FULL);
+  launch_update_tiled(s, b, Wstar, F, Win, Wout, st, H_FULL, false, false);
+}
+
+}  // namespace nmhd
+
 namespace {
 std::string g_adj_create_error;
 constexpr int kWork = 5 + 3 + 3 + 3 + 3 + 42;  // gU, gB, lE, lbb, abn, face
@@ -74,17 +93,8 @@ FluxSet flux_set(nmhd_adj *a) {
 
 // W_out = S(W_in; dt) with the production stage kernels.
 void enqueue_forward(nmhd_adj *a, FieldPtrs Win, FieldPtrs Wstar, FieldPtrs Wout, float dt) {
-  const FluxSet F = flux_set(a);
-  launch_adj_set_dt(a->stream, a->st, dt);
-  if (a->desc.integrator == NMHD_EULER1) {
-    launch_faces(a->stream, a->box, Win, F, a->st, H_FULL);
-    launch_update_tiled(a->stream, a->box, Win, F, Win, Wout, a->st, H_FULL, false, false);
-    return;
-  }
-  launch_faces(a->stream, a->box, Win, F, a->st, H_HALF);
-  launch_update_tiled(a->stream, a->box, Win, F, Win, Wstar, a->st, H_HALF, false, false);
-  launch_faces(a->stream, a->box, Wstar, F, a->st, H_FULL);
-  launch_update_tiled(a->stream, a->box, Wstar, F, Win, Wout, a->st, H_FULL, false, false);
+  enqueue_frozen_step(a->stream, a->box, a->desc.integrator, Win, Wstar, Wout, flux_set(a), a->st,
+                      dt);
 }
 
 AdjStageArgs stage_args(nmhd_adj *a) {
@@ -121,6 +131,10 @@ bool ptrs_ok(const float *const p[8]) {
 }
 
 }  // namespace
+
+namespace nmhd {
+void set_adj_global_error(const std::string &msg) { g_adj_create_error = msg; }
+}  // namespace nmhd
 
 extern "C" {
 

@@ -92,3 +92,23 @@ def test_create_rejects_every_invalid_descriptor_without_gpu():
         rc = L.nmhd_create(ctypes.byref(d), ctypes.byref(h))
         assert rc == 1 and not h.value, name
         assert msg in L.nmhd_last_error(None), (name, L.nmhd_last_error(None))
+
+
+def test_noalloc_step_rejects_before_any_device_call():
+    """nmhd_adj_forward_noalloc checks its descriptor, the box size against
+    its fixed scratch (2^20 cells) and aliasing before it touches a device;
+    the pointers below are never dereferenced."""
+    import ctypes
+    from nmhd_amd import adjoint as A, problems as P
+    L = A._lib()
+    w_in = (ctypes.c_void_p * 8)(*[0x1000 * (i + 1) for i in range(8)])
+    w_out = (ctypes.c_void_p * 8)(*[0x100000 * (i + 1) for i in range(8)])
+    cases = [
+        (P.make_config((128, 128, 65)), w_out, b"exceeds the no-allocation scratch"),
+        (P.config_c2((16, 16, 16)), w_out, b"periodic boxes"),
+        (P.make_config((16, 16, 16)), w_in, b"alias"),
+    ]
+    for cfg, out, msg in cases:
+        d = capi.desc_from_config(cfg)
+        assert L.nmhd_adj_forward_noalloc(ctypes.byref(d), w_in, 1e-3, out, None) == 1
+        assert msg in L.nmhd_adj_last_error(None)
