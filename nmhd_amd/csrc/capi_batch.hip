@@ -30,7 +30,7 @@ namespace {
 
 std::string g_batch_create_error;
 
-constexpr int kZChunks[3] = {8, 16, 32};
+constexpr int kZChunks[5] = {2, 4, 8, 16, 32};
 constexpr int kWgPerCu[5] = {0, 1, 2, 3, 4};  // 0: one workgroup per item
 constexpr int kTuneItems = 48;  // work items timed per shape and tile shape
 

@@ -52,12 +52,17 @@ int tiled_zchunk(const Box &b, int tx, int ty, bool faces) {
   // update is fastest with 16-plane chunks at both sizes; the face march
   // with 32 planes once that gives at least 4096 workgroups (4 per CU
   // resident: enough rounds that the last one is short), else 16.  Smaller
-  // chunks balance the grid better but repeat more prologue planes.  Never
-  // below 8 planes; shorter still (down to 8) while under 2048 workgroups.
+  // chunks balance the grid better but repeat more prologue planes.  Small
+  // boxes halve the chunk until the grid fills the chip: one full round of
+  // 4 workgroups per CU (1024) for marches of 8 or more planes, half of it
+  // for shorter ones, whose prologue planes weigh more.  Measured step
+  // times (periodic cubes, both kernels at the same chunk): 32^3 best at 1
+  // plane (2.7x faster than 8), 48^3 at 1, 64^3 at 2 (1.8x faster than 8;
+  // 1 plane is 6% slower), 96^3 at 4, 128^3 at 8 (16 is 12% slower).
   const long cols = (long)((b.n[0] + tx - 1) / tx) * ((b.n[1] + ty - 1) / ty);
   auto groups = [&](int zc) { return cols * ((b.n[2] + zc - 1) / zc); };
   int zc = faces ? (groups(32) >= 4096 ? 32 : 16) : 16;
-  while (zc > 8 && groups(zc) < 2048) zc /= 2;
+  while (zc > 1 && groups(zc) < (zc >= 8 ? 1024 : 512)) zc /= 2;
   return zc;
 }
 

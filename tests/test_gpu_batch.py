@@ -103,6 +103,8 @@ def test_batch_graph_equals_eager(gpu_ready):
     (capi.TILE_32x8, capi.TILE_64x4, 8, 16, 0, 0),
     (capi.TILE_64x4, capi.TILE_16x16, 3, 8, 0, 3),
     (capi.TILE_16x16, capi.TILE_32x8, 16, 5, 2, 0),
+    (capi.TILE_32x8, capi.TILE_64x4, 1, 2, 0, 0),
+    (capi.TILE_16x16, capi.TILE_32x8, 2, 1, 2, 2),
 ])
 def test_batch_tilings_bit_identical(gpu_ready, ft, ut, fz, uz, fw, uw):
     announce("batch tiling faces %d update %d zc %d/%d wg %d/%d" % (ft, ut, fz, uz, fw, uw))
