@@ -323,3 +323,61 @@ def rollout(W0, stepper, nsteps=None, dts=None, checkpoint="uniform", every=None
         snapshots = max(1, int(math.ceil(math.log2(max(n, 2)))))
     return _Rollout.apply(W0.contiguous(), stepper, tuple(dts), checkpoint, every,
                           snapshots), list(dts)
+
+
+def finite_difference_check(stepper, W0, dts, directions, grad, loss, eps=1e-3, nperturb=8,
+                            seed=5):
+    """Central differences of `loss` over the rollout of `dts` from W0 along
+    each direction, against the adjoint's directional derivatives grad . D.
+
+    The FP32 forward makes every evaluation of the loss carry round-off
+    noise: the perturbed inputs W0 +- eps D are rounded to FP32, and the
+    step's own rounding responds to any change of its input.  sigma, the
+    standard deviation of that noise in a central difference, is measured:
+    the loss at nperturb copies of W0 moved by random amounts of at most
+    half an ulp per value, spread s, gives sigma = s sqrt(2) / (2 eps).  A
+    central difference cannot resolve a derivative better than a few sigma,
+    so the per-direction criterion is |fd - ad| <= 1e-3 |ad| + 4 sigma: a
+    plain relative error of at most 1e-3 on every direction whose
+    derivative exceeds 4 sigma / 1e-3, and agreement to the noise on the
+    others.
+
+    directions: float64 CUDA tensors shaped like W0; grad: dloss/dW0 (float64);
+    loss: a function of the final state returning a float.
+    Returns a dict of per-direction lists and the summary."""
+    import numpy as np
+
+    def run(Wi):
+        X = Wi
+        with torch.no_grad():
+            for dt in dts:
+                X = stepper.forward(X, dt)
+        return loss(X)
+
+    Wd = W0.detach().double()
+    L0 = run(W0.detach())
+    gen = torch.Generator(device=W0.device).manual_seed(seed)
+    spread = []
+    for _ in range(nperturb):
+        r = torch.rand(W0.shape, generator=gen, device=W0.device, dtype=torch.float64) * 2 - 1
+        spread.append(run((Wd * (1 + r * 2.0 ** -24)).float().contiguous()) - L0)
+    sigma = float(np.std(spread)) * math.sqrt(2.0) / (2 * eps)
+    fd, ad = [], []
+    for D in directions:
+        fp = run((Wd + eps * D).float().contiguous())
+        fm = run((Wd - eps * D).float().contiguous())
+        fd.append((fp - fm) / (2 * eps))
+        ad.append(float((grad * D).sum()))
+    fd, ad = np.array(fd), np.array(ad)
+    err = np.abs(fd - ad)
+    rel = err / np.maximum(np.abs(ad), 1e-300)
+    resolvable = np.abs(ad) >= 4 * sigma / 1e-3
+    return {
+        "sigma": sigma, "fd": fd.tolist(), "ad": ad.tolist(), "rel_err": rel.tolist(),
+        "err_over_sigma": (err / max(sigma, 1e-300)).tolist(),
+        "ok": bool(np.all(err <= 1e-3 * np.abs(ad) + 4 * sigma)),
+        "resolvable": int(resolvable.sum()),
+        "rel_err_max_resolvable": float(rel[resolvable].max()) if resolvable.any() else None,
+        "rel_err_median": float(np.median(rel)),
+        "err_over_rms_derivative_max": float(err.max() / np.sqrt((ad ** 2).mean())),
+    }

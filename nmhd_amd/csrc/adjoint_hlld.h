@@ -21,6 +21,21 @@
 namespace nmhd {
 namespace adj {
 
+// Division, reciprocal and square root of the tape and the reverse sweep: on
+// the device the hardware reciprocal and square root (1 ulp), the arithmetic
+// of the production HLLD, instead of the IEEE sequences (a tenth of the
+// instructions); on the host (the double build that checks the reverse
+// sweep against finite differences) the IEEE operations.
+__device__ inline float qdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ inline float qrcp(float a) { return __builtin_amdgcn_rcpf(a); }
+__device__ inline float qsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+__host__ inline float qdiv(float a, float b) { return a / b; }
+__host__ inline float qrcp(float a) { return 1.0f / a; }
+__host__ inline float qsqrt(float x) { return std::sqrt(x); }
+__host__ __device__ inline double qdiv(double a, double b) { return a / b; }
+__host__ __device__ inline double qrcp(double a) { return 1.0 / a; }
+__host__ __device__ inline double qsqrt(double x) { return std::sqrt(x); }
+
 template <class T>
 struct HIn {
   T bx, rL, rR, unL, unR, u1L, u1R, u2L, u2R, pL, pR, b1L, b1R, b2L, b2R;
@@ -70,12 +85,12 @@ __host__ __device__ inline void hlld_forward(T gamma, T ig, const HIn<T> &in, HT
   t.gpbR = t.gpR + B2R;
   const T discL = t.gpbL * t.gpbL - T(4) * t.gpL * t.bx2;
   const T discR = t.gpbR * t.gpbR - T(4) * t.gpR * t.bx2;
-  t.sdL = sqrt(discL > T(0) ? discL : T(0));
-  t.sdR = sqrt(discR > T(0) ? discR : T(0));
-  t.cf2L = (t.gpbL + t.sdL) / (T(2) * in.rL);
-  t.cf2R = (t.gpbR + t.sdR) / (T(2) * in.rR);
-  t.cfL = sqrt(t.cf2L);
-  t.cfR = sqrt(t.cf2R);
+  t.sdL = qsqrt(discL > T(0) ? discL : T(0));
+  t.sdR = qsqrt(discR > T(0) ? discR : T(0));
+  t.cf2L = qdiv(t.gpbL + t.sdL, T(2) * in.rL);
+  t.cf2R = qdiv(t.gpbR + t.sdR, T(2) * in.rR);
+  t.cfL = qsqrt(t.cf2L);
+  t.cfR = qsqrt(t.cf2R);
   t.useL_c = t.cfL >= t.cfR;
   const T cmax = SQ2 * (t.useL_c ? t.cfL : t.cfR);
   t.minL = in.unL <= in.unR;
@@ -85,7 +100,7 @@ __host__ __device__ inline void hlld_forward(T gamma, T ig, const HIn<T> &in, HT
   t.sLu = t.SL - in.unL;
   t.sRu = t.SR - in.unR;
   t.den = t.sRu * in.rR - t.sLu * in.rL;
-  t.iden = T(1) / t.den;
+  t.iden = qrcp(t.den);
   t.SMn = t.sRu * in.rR * in.unR - t.sLu * in.rL * in.unL - t.ptR + t.ptL;
   t.SM = t.SMn * t.iden;
   t.ptsn = t.sRu * in.rR * t.ptL - t.sLu * in.rL * t.ptR +
@@ -93,14 +108,14 @@ __host__ __device__ inline void hlld_forward(T gamma, T ig, const HIn<T> &in, HT
   t.pts = t.ptsn * t.iden;
   t.sLm = t.SL - t.SM;
   t.sRm = t.SR - t.SM;
-  t.isLm = T(1) / t.sLm;
-  t.isRm = T(1) / t.sRm;
+  t.isLm = qrcp(t.sLm);
+  t.isRm = qrcp(t.sRm);
   t.rLs = in.rL * t.sLu * t.isLm;
   t.rRs = in.rR * t.sRu * t.isRm;
   t.dL = in.rL * t.sLu * t.sLm - t.bx2 + TINY;
   t.dR = in.rR * t.sRu * t.sRm - t.bx2 + TINY;
-  t.idL = T(1) / t.dL;
-  t.idR = T(1) / t.dR;
+  t.idL = qrcp(t.dL);
+  t.idR = qrcp(t.dR);
   t.kvL = bx * (t.SM - in.unL) * t.idL;
   t.kvR = bx * (t.SM - in.unR) * t.idR;
   t.kbLn = in.rL * t.sLu * t.sLu - t.bx2;
@@ -123,16 +138,16 @@ __host__ __device__ inline void hlld_forward(T gamma, T ig, const HIn<T> &in, HT
   t.eRsn = t.sRu * t.eR - t.ptR * in.unR + t.pts * t.SM + bx * (t.vbR - t.vbRs);
   t.eLs = t.eLsn * t.isLm;
   t.eRs = t.eRsn * t.isRm;
-  t.srL = sqrt(t.rLs);
-  t.srR = sqrt(t.rRs);
-  t.SLs = t.SM - t.abx / t.srL;
-  t.SRs = t.SM + t.abx / t.srR;
-  t.f = T(1) / (t.srL + t.srR);
+  t.srL = qsqrt(t.rLs);
+  t.srR = qsqrt(t.rRs);
+  t.SLs = t.SM - qdiv(t.abx, t.srL);
+  t.SRs = t.SM + qdiv(t.abx, t.srR);
+  t.f = qrcp(t.srL + t.srR);
   t.u1ssn = t.srL * t.u1Ls + t.srR * t.u1Rs + (t.b1Rs - t.b1Ls) * t.sgn;
   t.u2ssn = t.srL * t.u2Ls + t.srR * t.u2Rs + (t.b2Rs - t.b2Ls) * t.sgn;
   t.u1ss = t.f * t.u1ssn;
   t.u2ss = t.f * t.u2ssn;
-  t.srLR = sqrt(t.rLs * t.rRs);
+  t.srLR = qsqrt(t.rLs * t.rRs);
   t.b1ssn = t.srL * t.b1Rs + t.srR * t.b1Ls + t.srLR * (t.u1Rs - t.u1Ls) * t.sgn;
   t.b2ssn = t.srL * t.b2Rs + t.srR * t.b2Ls + t.srLR * (t.u2Rs - t.u2Ls) * t.sgn;
   t.b1ss = t.f * t.b1ssn;
@@ -325,7 +340,7 @@ __host__ __device__ inline void hlld_reverse(T gamma, T ig, const HIn<T> &in, co
   }
   // srLR = sqrt(rLs rRs)
   if (t.srLR > Z) {
-    const T g = a_srLR / (T(2) * t.srLR);
+    const T g = qdiv(a_srLR, T(2) * t.srLR);
     a_rLs += g * t.rRs;
     a_rRs += g * t.rLs;
   }
@@ -349,12 +364,12 @@ __host__ __device__ inline void hlld_reverse(T gamma, T ig, const HIn<T> &in, co
   a_srR -= a_f * t.f * t.f;
   // SLs = SM - abx/srL; SRs = SM + abx/srR
   a_SM += a_SLs + a_SRs;
-  a_abx += -a_SLs / t.srL + a_SRs / t.srR;
-  a_srL += a_SLs * t.abx / (t.srL * t.srL);
-  a_srR -= a_SRs * t.abx / (t.srR * t.srR);
+  a_abx += -qdiv(a_SLs, t.srL) + qdiv(a_SRs, t.srR);
+  a_srL += qdiv(a_SLs * t.abx, t.srL * t.srL);
+  a_srR -= qdiv(a_SRs * t.abx, t.srR * t.srR);
   // srL = sqrt(rLs)
-  a_rLs += a_srL / (T(2) * t.srL);
-  a_rRs += a_srR / (T(2) * t.srR);
+  a_rLs += qdiv(a_srL, T(2) * t.srL);
+  a_rRs += qdiv(a_srR, T(2) * t.srR);
 
   // ---- single-star states
   T a_sLu = Z, a_sRu = Z, a_isLm = Z, a_isRm = Z, a_pts = Z;
@@ -526,14 +541,14 @@ __host__ __device__ inline void hlld_reverse(T gamma, T ig, const HIn<T> &in, co
     a_cfR += SQ2 * a_cmax;
   T a_gpL = Z, a_gpR = Z, a_B2L = Z, a_B2R = Z;
   {  // cfL = sqrt(cf2L), cf2L = (gpbL + sdL)/(2 rL), sdL = sqrt(gpbL^2 - 4 gpL bx2)
-    const T a_cf2L = t.cfL > Z ? a_cfL / (T(2) * t.cfL) : Z;
-    const T a_cf2R = t.cfR > Z ? a_cfR / (T(2) * t.cfR) : Z;
-    T a_gpbL = a_cf2L / (T(2) * in.rL), a_gpbR = a_cf2R / (T(2) * in.rR);
-    const T a_sdL = a_cf2L / (T(2) * in.rL), a_sdR = a_cf2R / (T(2) * in.rR);
-    a_rL -= a_cf2L * t.cf2L / in.rL;
-    a_rR -= a_cf2R * t.cf2R / in.rR;
-    const T a_discL = t.sdL > Z ? a_sdL / (T(2) * t.sdL) : Z;
-    const T a_discR = t.sdR > Z ? a_sdR / (T(2) * t.sdR) : Z;
+    const T a_cf2L = t.cfL > Z ? qdiv(a_cfL, T(2) * t.cfL) : Z;
+    const T a_cf2R = t.cfR > Z ? qdiv(a_cfR, T(2) * t.cfR) : Z;
+    T a_gpbL = qdiv(a_cf2L, T(2) * in.rL), a_gpbR = qdiv(a_cf2R, T(2) * in.rR);
+    const T a_sdL = a_gpbL, a_sdR = a_gpbR;
+    a_rL -= qdiv(a_cf2L * t.cf2L, in.rL);
+    a_rR -= qdiv(a_cf2R * t.cf2R, in.rR);
+    const T a_discL = t.sdL > Z ? qdiv(a_sdL, T(2) * t.sdL) : Z;
+    const T a_discR = t.sdR > Z ? qdiv(a_sdR, T(2) * t.sdR) : Z;
     a_gpbL += a_discL * T(2) * t.gpbL;
     a_gpbR += a_discR * T(2) * t.gpbR;
     a_gpL -= a_discL * T(4) * t.bx2;

@@ -36,7 +36,7 @@ void enqueue_frozen_step(hipStream_t s, const Box &b, int integrator, FieldPtrs 
 
 namespace {
 std::string g_adj_create_error;
-constexpr int kWork = 5 + 3 + 3 + 3 + 3 + 42;  // gU, gB, lE, lbb, abn, face
+constexpr int kWork = 5 + 3 + 3 + 3 + 42;  // gU, lE, lbb, abn, face
 }  // namespace
 
 struct nmhd_adj {
@@ -102,7 +102,6 @@ AdjStageArgs stage_args(nmhd_adj *a) {
   std::memset(&A, 0, sizeof(A));
   float **w = a->work;
   for (int i = 0; i < 5; ++i) A.gU[i] = *w++;
-  for (int i = 0; i < 3; ++i) A.gB[i] = *w++;
   for (int i = 0; i < 3; ++i) A.lE[i] = *w++;
   for (int i = 0; i < 3; ++i) A.lbb[i] = *w++;
   for (int i = 0; i < 3; ++i) A.abn[i] = *w++;

@@ -8,10 +8,11 @@
 //
 // Kernels of one stage reverse (all gathers: every thread writes only its
 // own cell, no atomics, results independent of scheduling):
-//   k_adj_prim    adjoint of prim(U) and of cons(W_base): gU (5 per cell),
-//                 gB (3 face values per cell) and lam_base
-//   k_adj_edges   adjoint of the 6 CT edge-EMF families (3 distinct values per
-//                 cell: lamE_x, lamE_y, lamE_z)
+//   k_adj_prim    adjoint of prim(U) and of cons(W_base): gU (5 per cell)
+//                 and lam_base; and the adjoints of the 6 CT edge-EMF families
+//                 (3 distinct values per cell: lamE_x, lamE_y, lamE_z) from the
+//                 face-B adjoints of the cell and its lower neighbours, which
+//                 the thread recomputes rather than reading them back
 //   k_adj_faces   per face of the three directions (blockIdx.y): adjoint of
 //                 the 7 fluxes (divergence + upwinded EMFs, with the forward's
 //                 upwind choice recomputed), then the HLLD reverse sweep
@@ -79,11 +80,24 @@ __device__ __forceinline__ float edge_sum(const Box &b, const float *v, int x, i
 
 // ------------------------------------------------------------- k_adj_prim
 
+// Adjoint of the face value B_d at cell c (its lower d-face) from the stage
+// output: its own adjoint, and through the pressure of the two cells that
+// share the face (c and its lower d-neighbour m).
+__device__ __forceinline__ float face_b_adj(const Box &b, const FieldPtrs &Wout,
+                                            const FieldPtrs &lam, int d, long c, long m) {
+  return lam.f[5 + d][c] - b.gm1 * 0.5f * Wout.f[5 + d][c] * (lam.f[4][c] + lam.f[4][m]);
+}
+
+// lamE_z = adjoint of the z-edge EMF from the x-sweep (= minus that of the
+// y-sweep's), at the edge on the lower x- and y-faces of the cell; likewise
+// lamE_y (x-sweep / z-sweep pair on the lower x- and z-faces) and lamE_x
+// (y-sweep / z-sweep pair on the lower y- and z-faces), scaled by h.
 template <bool AccBase>
 __global__ void __launch_bounds__(256) k_adj_prim(Box b, FieldPtrs Wout, FieldPtrs Wbase,
                                                   FieldPtrs lam, float *gU0, float *gU1,
-                                                  float *gU2, float *gU3, float *gU4, float *gB0,
-                                                  float *gB1, float *gB2, FieldPtrs lam_base) {
+                                                  float *gU2, float *gU3, float *gU4, float h,
+                                                  float *lEx, float *lEy, float *lEz,
+                                                  FieldPtrs lam_base) {
   int x, y, z;
   long a;
   if (!cell_of(b, x, y, z, a)) return;
@@ -91,7 +105,7 @@ __global__ void __launch_bounds__(256) k_adj_prim(Box b, FieldPtrs Wout, FieldPt
   const float rho = Wout.f[0][a];
   const float v[3] = {Wout.f[1][a], Wout.f[2][a], Wout.f[3][a]};
   const float lp = lam.f[4][a];
-  const float irho = 1.0f / rho;
+  const float irho = frcp(rho);
   const float gE = gm1 * lp;
   float gm[3];
   float grho = lam.f[0][a] + gm1 * lp * 0.5f * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
@@ -99,21 +113,29 @@ __global__ void __launch_bounds__(256) k_adj_prim(Box b, FieldPtrs Wout, FieldPt
     gm[d] = lam.f[1 + d][a] * irho - gm1 * lp * v[d];
     grho -= lam.f[1 + d][a] * v[d] * irho;
   }
-  // face B_d (lower d-face): its own adjoint, and through the pressure of
-  // the two cells that share it
+  const long xm = nb(b, x, y, z, 0, -1), ym = nb(b, x, y, z, 1, -1), zm = nb(b, x, y, z, 2, -1);
+  const long mm[3] = {xm, ym, zm};
   float gB[3];
-  for (int d = 0; d < 3; ++d) {
-    const long m = nb(b, x, y, z, d, -1);
-    gB[d] = lam.f[5 + d][a] - gm1 * 0.5f * Wout.f[5 + d][a] * (lp + lam.f[4][m]);
-  }
+  for (int d = 0; d < 3; ++d) gB[d] = face_b_adj(b, Wout, lam, d, a, mm[d]);
   gU0[a] = grho;
   gU1[a] = gm[0];
   gU2[a] = gm[1];
   gU3[a] = gm[2];
   gU4[a] = gE;
-  gB0[a] = gB[0];
-  gB1[a] = gB[1];
-  gB2[a] = gB[2];
+  // the edge-EMF adjoints: face-B adjoints of the lower x-, y- and
+  // z-neighbours, each through its own lower neighbour along its face axis
+  {
+    const float idx = b.inv_d[0], idy = b.inv_d[1], idz = b.inv_d[2];
+    const float gB0_ym = face_b_adj(b, Wout, lam, 0, ym, nb(b, x, wrap(y - 1, b.n[1]), z, 0, -1));
+    const float gB0_zm = face_b_adj(b, Wout, lam, 0, zm, nb(b, x, y, wrap(z - 1, b.n[2]), 0, -1));
+    const float gB1_xm = face_b_adj(b, Wout, lam, 1, xm, nb(b, wrap(x - 1, b.n[0]), y, z, 1, -1));
+    const float gB1_zm = face_b_adj(b, Wout, lam, 1, zm, nb(b, x, y, wrap(z - 1, b.n[2]), 1, -1));
+    const float gB2_xm = face_b_adj(b, Wout, lam, 2, xm, nb(b, wrap(x - 1, b.n[0]), y, z, 2, -1));
+    const float gB2_ym = face_b_adj(b, Wout, lam, 2, ym, nb(b, x, wrap(y - 1, b.n[1]), z, 2, -1));
+    lEz[a] = h * (0.5f * idy * (gB0_ym - gB[0]) + 0.5f * idx * (gB[1] - gB1_xm));
+    lEy[a] = h * (0.5f * idz * (gB0_zm - gB[0]) + 0.5f * idx * (gB[2] - gB2_xm));
+    lEx[a] = h * (0.5f * idz * (gB1_zm - gB[1]) + 0.5f * idy * (gB[2] - gB2_ym));
+  }
   // cons(W_base): rho, m = rho v, E = p/(g-1) + rho v^2/2 + (B_lo^2 + B_hi^2)/4
   const float rho0 = Wbase.f[0][a];
   const float v0[3] = {Wbase.f[1][a], Wbase.f[2][a], Wbase.f[3][a]};
@@ -124,30 +146,9 @@ __global__ void __launch_bounds__(256) k_adj_prim(Box b, FieldPtrs Wout, FieldPt
     out[1 + d] = gm[d] * rho0 + gE * rho0 * v0[d];
   }
   out[4] = gE * ig;
-  for (int d = 0; d < 3; ++d) {
-    const long m = nb(b, x, y, z, d, -1);
-    out[5 + d] = gB[d] + 0.5f * Wbase.f[5 + d][a] * (gE + gm1 * lam.f[4][m]);
-  }
+  for (int d = 0; d < 3; ++d)
+    out[5 + d] = gB[d] + 0.5f * Wbase.f[5 + d][a] * (gE + gm1 * lam.f[4][mm[d]]);
   for (int f = 0; f < 8; ++f) lam_base.f[f][a] = AccBase ? lam_base.f[f][a] + out[f] : out[f];
-}
-
-// ------------------------------------------------------------ k_adj_edges
-//
-// lamE_z = adjoint of the z-edge EMF from the x-sweep (= minus that of the
-// y-sweep's), at the edge on the lower x- and y-faces of the cell; likewise
-// lamE_y (x-sweep / z-sweep pair on the lower x- and z-faces) and lamE_x
-// (y-sweep / z-sweep pair on the lower y- and z-faces).  gB is scaled by h.
-__global__ void __launch_bounds__(256) k_adj_edges(Box b, const float *gB0, const float *gB1,
-                                                   const float *gB2, float h, float *lEx,
-                                                   float *lEy, float *lEz) {
-  int x, y, z;
-  long a;
-  if (!cell_of(b, x, y, z, a)) return;
-  const float idx = b.inv_d[0], idy = b.inv_d[1], idz = b.inv_d[2];
-  const long xm = nb(b, x, y, z, 0, -1), ym = nb(b, x, y, z, 1, -1), zm = nb(b, x, y, z, 2, -1);
-  lEz[a] = h * (0.5f * idy * (gB0[ym] - gB0[a]) + 0.5f * idx * (gB1[a] - gB1[xm]));
-  lEy[a] = h * (0.5f * idz * (gB0[zm] - gB0[a]) + 0.5f * idx * (gB2[a] - gB2[xm]));
-  lEx[a] = h * (0.5f * idz * (gB1[zm] - gB1[a]) + 0.5f * idy * (gB2[a] - gB2[ym]));
 }
 
 // ------------------------------------------------------------- k_adj_face
@@ -420,12 +421,12 @@ void launch_adj_stage(hipStream_t s, const Box &b, const AdjStageArgs &A) {
   const dim3 g(grid_for(b.ncell)), blk(kBlock);
   if (A.acc_base)
     hipLaunchKernelGGL(k_adj_prim<true>, g, blk, 0, s, b, A.Wout, A.Wbase, A.lam_out, A.gU[0],
-                       A.gU[1], A.gU[2], A.gU[3], A.gU[4], A.gB[0], A.gB[1], A.gB[2], A.lam_base);
+                       A.gU[1], A.gU[2], A.gU[3], A.gU[4], A.h, A.lE[0], A.lE[1], A.lE[2],
+                       A.lam_base);
   else
     hipLaunchKernelGGL(k_adj_prim<false>, g, blk, 0, s, b, A.Wout, A.Wbase, A.lam_out, A.gU[0],
-                       A.gU[1], A.gU[2], A.gU[3], A.gU[4], A.gB[0], A.gB[1], A.gB[2], A.lam_base);
-  hipLaunchKernelGGL(k_adj_edges, g, blk, 0, s, b, A.gB[0], A.gB[1], A.gB[2], A.h, A.lE[0],
-                     A.lE[1], A.lE[2]);
+                       A.gU[1], A.gU[2], A.gU[3], A.gU[4], A.h, A.lE[0], A.lE[1], A.lE[2],
+                       A.lam_base);
   AdjFaceOut3 fo;
   for (int d = 0; d < 3; ++d) {
     for (int k = 0; k < 7; ++k) {

@@ -203,7 +203,7 @@ struct AdjStageArgs {
   float h;
   bool acc_base, acc_in;
   // work arrays (ncell floats each); face: 14 per direction (aL[7], aR[7])
-  float *gU[5], *gB[3], *lE[3], *lbb[3], *abn[3], *face[42];
+  float *gU[5], *lE[3], *lbb[3], *abn[3], *face[42];
 };
 void launch_adj_set_dt(hipStream_t s, StepState *st, float dt);
 void launch_adj_stage(hipStream_t s, const Box &b, const AdjStageArgs &A);
