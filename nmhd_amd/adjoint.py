@@ -18,9 +18,10 @@ backward pass:
   "all"      every state and every step's midpoint stage state (memory
              2N, no recomputation);
   "uniform"  every `every`-th state (default round(sqrt(N))): memory
-             N/every + 2 every, one recomputed forward pass that keeps the
+             N/every + 4 every, one recomputed forward pass that keeps the
              segment's stage states, so the reverse steps run no forward
-             kernels;
+             kernels; the recomputation of the next segment runs on a side
+             stream while the current one is reversed;
   "revolve"  binomial checkpointing with `snapshots` slots (default
              ceil(log2 N)): the split points minimise the number of
              recomputed steps for that memory (the offline optimum that
@@ -100,9 +101,22 @@ class Stepper:
                                  % (capi.ERRORS.get(rc, rc), msg.decode() if msg else ""))
 
     def close(self):
+        twin = getattr(self, "_twin", None)
+        if twin is not None:
+            twin.close()
+            self._twin = None
         if getattr(self, "h", None):
             _lib().nmhd_adj_destroy(self.h)
             self.h = None
+
+    def side(self):
+        """(side stream, second stepper of the same box) for work that runs
+        beside this stepper's: the checkpointed backward pass recomputes the
+        next segment there.  Both are made once and kept."""
+        if getattr(self, "_twin", None) is None:
+            self._twin = Stepper(self.cfg)
+            self._side_stream = torch.cuda.Stream()
+        return self._side_stream, self._twin
 
     def __del__(self):
         try:
@@ -267,17 +281,40 @@ class _Rollout(torch.autograd.Function):
             for j in range(n - 1, -1, -1):
                 lam = stepper.reverse_stages(keep[j], ctx.stage[j], keep[j + 1], dts[j], lam)
         else:
-            # one forward pass over the segment keeps every step's stage
-            # states, so no step's reverse runs the forward kernels again
+            # one forward pass over each segment keeps every step's stage
+            # states, so no step's reverse runs the forward kernels again.
+            # The recomputation of segment i-1 does not depend on the
+            # reverse of segment i: it runs on a side stream (default
+            # priority) with a second stepper, whose scratch is its own,
+            # while segment i is reversed on the caller's stream.
             starts = sorted(keep)
-            for i in range(len(starts) - 1, -1, -1):
+            main = torch.cuda.current_stream()
+            side, helper = stepper.side()
+            side.wait_stream(main)  # the checkpoints are the forward's outputs
+
+            def recompute(i):
                 a = starts[i]
                 b = starts[i + 1] if i + 1 < len(starts) else n
-                states, stages = [keep[a]], []
-                for j in range(a, b):
-                    ws, w = stepper.forward_stages(states[-1], dts[j])
-                    stages.append(ws)
-                    states.append(w)
+                with torch.cuda.stream(side):
+                    states, stages = [keep[a]], []
+                    for j in range(a, b):
+                        ws, w = helper.forward_stages(states[-1], dts[j])
+                        stages.append(ws)
+                        states.append(w)
+                    done = torch.cuda.Event()
+                    done.record(side)
+                return a, b, states, stages, done
+
+            pending = recompute(len(starts) - 1)
+            for i in range(len(starts) - 1, -1, -1):
+                a, b, states, stages, done = pending
+                main.wait_event(done)
+                if i > 0:
+                    pending = recompute(i - 1)
+                # made on the side stream, read on this one: not reused
+                # before this stream's work on them is done
+                for t in states[1:] + [t for t in stages if t is not None]:
+                    t.record_stream(main)
                 for j in range(b - 1, a - 1, -1):
                     lam = stepper.reverse_stages(states[j - a], stages[j - a],
                                                  states[j - a + 1], dts[j], lam)

@@ -86,6 +86,19 @@ __device__ __forceinline__ void recon(int mode, float x0, float x1, float x2, fl
   }
 }
 
+// Reconstruction of a cell-centred B component, the average of its cell's two
+// faces, from the four stencil cells' face sums a + b instead of the
+// averages 0.5 (a + b): every step of the reconstruction (differences,
+// median of three, the half-slope multiply-add) commutes exactly with the
+// scaling by 2, so halving the two face states gives the same bits as
+// reconstructing the averages, with 4 multiplies fewer per variable.
+__device__ __forceinline__ void recon_bsum(int mode, float s0, float s1, float s2, float s3,
+                                           float &l, float &r) {
+  recon(mode, s0, s1, s2, s3, l, r);
+  l *= 0.5f;
+  r *= 0.5f;
+}
+
 __device__ __forceinline__ float upwind(float s, float f0, float f1) {
   return s > 0.0f ? f0 : (s < 0.0f ? f1 : 0.5f * (f0 + f1));
 }
@@ -230,10 +243,10 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
     recon(mode, VY[s0], VY[s1], VY[s2], VY[s3], L[2], Rr[2]);
     recon(mode, VZ[s0], VZ[s1], VZ[s2], VZ[s3], L[3], Rr[3]);
     recon(mode, PR[s0], PR[s1], PR[s2], PR[s3], L[4], Rr[4]);
-    recon(mode, 0.5f * (BY[s0] + BY[s0 + LX]), 0.5f * (BY[s1] + BY[s1 + LX]),
-          0.5f * (BY[s2] + BY[s2 + LX]), 0.5f * (BY[s3] + BY[s3 + LX]), L[5], Rr[5]);
-    recon(mode, 0.5f * (BZ[s0] + BZN[s0]), 0.5f * (BZ[s1] + BZN[s1]), 0.5f * (BZ[s2] + BZN[s2]),
-          0.5f * (BZ[s3] + BZN[s3]), L[6], Rr[6]);
+    recon_bsum(mode, BY[s0] + BY[s0 + LX], BY[s1] + BY[s1 + LX], BY[s2] + BY[s2 + LX],
+               BY[s3] + BY[s3 + LX], L[5], Rr[5]);
+    recon_bsum(mode, BZ[s0] + BZN[s0], BZ[s1] + BZN[s1], BZ[s2] + BZN[s2], BZ[s3] + BZN[s3],
+               L[6], Rr[6]);
     if (src) {
       const float mbv = 0.5f * h * accel_x(b, 0.5f * (VY[s1] + VY[s2]));
       L[1] += mbv;
@@ -256,10 +269,10 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
     recon(mode, VZ[s0], VZ[s1], VZ[s2], VZ[s3], L[2], Rr[2]);
     recon(mode, VX[s0], VX[s1], VX[s2], VX[s3], L[3], Rr[3]);
     recon(mode, PR[s0], PR[s1], PR[s2], PR[s3], L[4], Rr[4]);
-    recon(mode, 0.5f * (BZ[s0] + BZN[s0]), 0.5f * (BZ[s1] + BZN[s1]), 0.5f * (BZ[s2] + BZN[s2]),
-          0.5f * (BZ[s3] + BZN[s3]), L[5], Rr[5]);
-    recon(mode, 0.5f * (BX[s0] + BX[s0 + 1]), 0.5f * (BX[s1] + BX[s1 + 1]),
-          0.5f * (BX[s2] + BX[s2 + 1]), 0.5f * (BX[s3] + BX[s3 + 1]), L[6], Rr[6]);
+    recon_bsum(mode, BZ[s0] + BZN[s0], BZ[s1] + BZN[s1], BZ[s2] + BZN[s2], BZ[s3] + BZN[s3],
+               L[5], Rr[5]);
+    recon_bsum(mode, BX[s0] + BX[s0 + 1], BX[s1] + BX[s1 + 1], BX[s2] + BX[s2 + 1],
+               BX[s3] + BX[s3 + 1], L[6], Rr[6]);
     if (src) {
       const float mbv = 0.5f * h * accel_y(b, 0.5f * (VX[s1] + VX[s2]), yrow);
       L[1] += mbv;
@@ -291,10 +304,9 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
     recon(mode, VY[s0], VY[s1], VY[s2], VY[s3], ly, ry);
     recon(mode, VZ[s0], VZ[s1], VZ[s2], VZ[s3], lz, rz);
     recon(mode, PR[s0], PR[s1], PR[s2], PR[s3], L[4], Rr[4]);
-    recon(mode, 0.5f * (OA[s0] + OB[s0]), 0.5f * (OA[s1] + OB[s1]), 0.5f * (OA[s2] + OB[s2]),
-          0.5f * (OA[s3] + OB[s3]), lo, ro);
-    recon(mode, 0.5f * (BZ[s0] + BZN[s0]), 0.5f * (BZ[s1] + BZN[s1]), 0.5f * (BZ[s2] + BZN[s2]),
-          0.5f * (BZ[s3] + BZN[s3]), lc, rc);
+    recon_bsum(mode, OA[s0] + OB[s0], OA[s1] + OB[s1], OA[s2] + OB[s2], OA[s3] + OB[s3], lo, ro);
+    recon_bsum(mode, BZ[s0] + BZN[s0], BZ[s1] + BZN[s1], BZ[s2] + BZN[s2], BZ[s3] + BZN[s3], lc,
+               rc);
     // x-face: (vx, vy, vz), (B_y, B_z); y-face: (vy, vz, vx), (B_z, B_x)
     L[1] = isy ? ly : lx, Rr[1] = isy ? ry : rx;
     L[2] = isy ? lz : ly, Rr[2] = isy ? rz : ry;
