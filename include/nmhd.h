@@ -117,8 +117,8 @@ typedef struct nmhd_ctx nmhd_ctx;
 int nmhd_create(const nmhd_desc *desc, nmhd_ctx **out);
 /* ~MHDSolver: free the context and its device memory. */
 int nmhd_destroy(nmhd_ctx *ctx);
-/* Message of the last failure on ctx (or of the last failed nmhd_create when
- * ctx is NULL). */
+/* Message of the last failure on ctx (or, when ctx is NULL, of the calling
+ * thread's last failed nmhd_create). */
 const char *nmhd_last_error(const nmhd_ctx *ctx);
 
 int nmhd_get_geometry(const nmhd_ctx *ctx, nmhd_geometry *geo);
@@ -231,8 +231,8 @@ typedef struct nmhd_batch nmhd_batch;
 
 int nmhd_batch_create(const nmhd_desc *descs, int nbox, nmhd_batch **out);
 int nmhd_batch_destroy(nmhd_batch *batch);
-/* Message of the last failure on batch (or of the last failed create when
- * batch is NULL). */
+/* Message of the last failure on batch (or, when batch is NULL, of the
+ * calling thread's last failed create). */
 const char *nmhd_batch_last_error(const nmhd_batch *batch);
 int nmhd_batch_get_geometry(const nmhd_batch *batch, int box, nmhd_geometry *geo);
 
@@ -292,8 +292,8 @@ typedef struct nmhd_adj nmhd_adj;
 
 int nmhd_adj_create(const nmhd_desc *desc, nmhd_adj **out);
 int nmhd_adj_destroy(nmhd_adj *adj);
-/* Message of the last failure on adj (or, for adj == NULL, of the last
- * failed nmhd_adj_create or nmhd_adj_forward_noalloc). */
+/* Message of the last failure on adj (or, for adj == NULL, of the calling
+ * thread's last failed nmhd_adj_create or nmhd_adj_forward_noalloc). */
 const char *nmhd_adj_last_error(const nmhd_adj *adj);
 int nmhd_adj_get_geometry(const nmhd_adj *adj, nmhd_geometry *geo);
 /* Launch later work on `stream`, a hipStream_t such as

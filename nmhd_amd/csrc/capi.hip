@@ -112,7 +112,8 @@ struct Pending {
   hipEvent_t a, b;
 };
 
-std::string g_create_error;
+// per thread: the message of the calling thread's last failed create
+thread_local std::string g_create_error;
 
 }  // namespace
 

@@ -35,7 +35,8 @@ void enqueue_frozen_step(hipStream_t s, const Box &b, int integrator, FieldPtrs 
 }  // namespace nmhd
 
 namespace {
-std::string g_adj_create_error;
+// per thread: the message of the calling thread's last object-less failure
+thread_local std::string g_adj_create_error;
 constexpr int kWork = 5 + 3 + 3 + 3 + 42;  // gU, lE, lbb, abn, face
 }  // namespace
 

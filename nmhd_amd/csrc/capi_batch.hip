@@ -28,7 +28,8 @@ using namespace nmhd;
 
 namespace {
 
-std::string g_batch_create_error;
+// per thread: the message of the calling thread's last failed create
+thread_local std::string g_batch_create_error;
 
 constexpr int kZChunks[5] = {2, 4, 8, 16, 32};
 constexpr int kWgPerCu[5] = {0, 1, 2, 3, 4};  // 0: one workgroup per item
