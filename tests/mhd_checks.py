@@ -83,3 +83,30 @@ def group_frac_within(a, b, tol, groups=STATE_GROUPS):
         scale = max(np.abs(b[g]).max(), 1e-30)
         ok &= (np.abs(a[g] - b[g]) <= tol * scale).all(axis=0)
     return ok.mean()
+
+
+def group_stats(a, b, tol, groups=STATE_GROUPS, zchunk=16):
+    """(group_rel_err(a, b), group_frac_within(a, b, tol)) evaluated slab by
+    slab along z, so that arrays of billions of values need no float64
+    copies of their own size."""
+    nz = b.shape[1]
+    slabs = [slice(z, min(z + zchunk, nz)) for z in range(0, nz, zchunk)]
+    scale = []
+    for g in groups:
+        m = 0.0
+        for f in g:
+            for s in slabs:
+                m = max(m, float(np.abs(b[f, s]).max()))
+        scale.append(max(m, 1e-30))
+    err = np.zeros(len(groups))
+    ok_cells = 0
+    for s in slabs:
+        ok = None
+        for gi, g in enumerate(groups):
+            for f in g:
+                d = np.abs(a[f, s].astype(np.float64) - b[f, s])
+                err[gi] = max(err[gi], float(d.max()))
+                within = d <= tol * scale[gi]
+                ok = within if ok is None else ok & within
+        ok_cells += int(ok.sum())
+    return err / np.array(scale), ok_cells / float(np.prod(b.shape[1:]))

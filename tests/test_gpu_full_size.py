@@ -14,8 +14,10 @@ through what does not depend on size:
 * a periodic 256^3 Orszag-Tang box: mass, momentum, energy and the face-B
   totals (FP64 sums on the device) conserved to 1e-6 relative over 20 steps
   (flux telescoping; a dropped or mis-signed flux term moves them by ~1e-3);
-* the 512x1024x512 beta-1000 box: 5 steps, div B at round-off and finite
-  totals (about 49 GB of device memory).
+* the 512x1024x512 beta-1000 box, the largest workload: one step on the
+  oracle against the strict path (bit for bit) and the tiled path (the
+  256^3 criteria); 5 steps with div B at round-off and finite totals (about
+  49 GB of device memory per context).
 """
 import math
 
@@ -23,7 +25,7 @@ import numpy as np
 import pytest
 import torch
 
-from mhd_checks import bitwise_equal, group_frac_within, group_rel_err, perturb_ulps
+from mhd_checks import bitwise_equal, group_frac_within, group_rel_err, group_stats, perturb_ulps
 from nmhd_amd import capi, problems as P
 
 pytestmark = pytest.mark.gpu
@@ -159,3 +161,56 @@ def test_c3_large_box_steps(gpu_ready):
     assert all(math.isfinite(v) for v in d["total"])
     assert d["total"][0] > 0
     g.close()
+
+
+def test_c3_large_box_one_step_vs_oracle(gpu_ready):
+    """The largest workload, 512x1024x512 (2.7e8 cells; 528,384-cell planes
+    through the kernels' 32-bit in-plane offsets), one step: the oracle on
+    the host's cores (about a minute) against the strict path, bit for bit
+    (dt, time, every cell including the ghost rows), and the tiled path: the
+    same dt to rounding, >= 99.9% of the cells within 1e-5 in every field
+    group, and no group further off than three times the strict path's own
+    change for a one-ulp change of its input (the strict path rounds as the
+    oracle does)."""
+    import pyoracle as po
+    announce("512x1024x512: one step on the oracle vs the strict and tiled paths")
+    cfg = P.config_c2((512, 1024, 512))
+    Wd = P.shearing_box_noise_device(cfg, seed=1003, beta=1000.0)
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    up = torch.full((), float("inf"), device="cuda")
+    Wp = torch.where(torch.rand(Wd.shape, generator=gen, device="cuda") < 0.5,
+                     torch.nextafter(Wd, up), Wd)
+    torch.cuda.synchronize()
+    out = {}
+    for name, path, src in (("strict", capi.PATH_STRICT, Wd), ("tiled", capi.PATH_TILED, Wd),
+                            ("strict_ulp", capi.PATH_STRICT, Wp)):
+        g = capi.Context(cfg, path=path)
+        g.set_state_device([src[i].data_ptr() for i in range(8)])
+        dt, _ = g.step()
+        out[name] = (dt, g.get_time()[0], g.get_state())
+        g.close()
+        print("%s path: dt %.8g" % (name, dt), flush=True)
+    W = Wd.cpu().numpy()
+    del Wd, Wp, up
+    torch.cuda.empty_cache()
+    print("the oracle's step on the host cores (about a minute)", flush=True)
+    o = po.Oracle(cfg, W)
+    del W
+    dt_o, _ = o.step()
+    t_o = o.time
+    ref = o.get_state()
+    o.close()
+    print("oracle: dt %.8g" % dt_o, flush=True)
+    dt, t, S = out.pop("strict")
+    assert dt == dt_o and t == t_o
+    assert bitwise_equal(S, ref)
+    del S
+    env, _ = group_stats(out.pop("strict_ulp")[2], ref, 1e-5)
+    dt, t, S = out.pop("tiled")
+    err, frac = group_stats(S, ref, 1e-5)
+    print("tiled vs oracle: group rel err %s, one-ulp envelope %s, fraction within 1e-5: %.6f"
+          % (np.array2string(err, precision=2), np.array2string(env, precision=2), frac),
+          flush=True)
+    assert abs(dt - dt_o) <= 1e-6 * dt_o
+    assert frac >= 0.999
+    assert np.all(err <= 3.0 * env + 1e-6), (err, env)
