@@ -28,9 +28,14 @@ The JSON line also carries
   adjoint      -- the hand-written adjoint on the 64^3 Orszag-Tang box with
                   a smooth density perturbation (seed 1005): 64 steps with dt
                   recorded from the forward pass, loss = magnetic energy,
-                  checkpoints every 8 steps; backward/forward time ratio and
-                  central finite differences (eps 1e-3) along 32 smooth
-                  random directions (seed 1006).
+                  checkpoints every 8 steps; backward/forward GPU-time ratio
+                  (HIP events, median of 5) and central finite differences
+                  (eps 1e-3) along 32 smooth random directions (seed 1006),
+                  per direction against a measured FP32 noise floor, on the
+                  same state in a bulk flow with piecewise-constant
+                  reconstruction (where the discrete step is differentiable
+                  along them), and, reported beside it, on the workload
+                  state itself.
 """
 import argparse
 import json

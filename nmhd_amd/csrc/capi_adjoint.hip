@@ -37,7 +37,7 @@ void enqueue_frozen_step(hipStream_t s, const Box &b, int integrator, FieldPtrs 
 namespace {
 // per thread: the message of the calling thread's last object-less failure
 thread_local std::string g_adj_create_error;
-constexpr int kWork = 5 + 3 + 3 + 3 + 42;  // gU, lE, lbb, abn, face
+constexpr int kWork = 5 + 3 + 3 + 3 + 84;  // gU, lE, lbb, abn, face
 }  // namespace
 
 struct nmhd_adj {
@@ -106,7 +106,7 @@ AdjStageArgs stage_args(nmhd_adj *a) {
   for (int i = 0; i < 3; ++i) A.lE[i] = *w++;
   for (int i = 0; i < 3; ++i) A.lbb[i] = *w++;
   for (int i = 0; i < 3; ++i) A.abn[i] = *w++;
-  for (int i = 0; i < 42; ++i) A.face[i] = *w++;
+  for (int i = 0; i < 84; ++i) A.face[i] = *w++;
   return A;
 }
 

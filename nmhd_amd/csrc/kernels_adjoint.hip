@@ -18,10 +18,11 @@
 //                 upwind choice recomputed), then the HLLD reverse sweep
 //                 (adjoint_hlld.h) -> adjoints of the 7 left/right face states
 //                 and of the normal field
-//   k_adj_gather  per cell: reverse of the minmod reconstruction of the three
-//                 directions from the 4 faces whose stencils hold the cell
-//                 (forward minmod branches recomputed) -> lam_in of rho, v, p
-//                 and of the cell-averaged B
+//   k_adj_gather  per cell: the reconstruction adjoints of the three
+//                 directions, summed from the 4 faces whose stencils hold the
+//                 cell (each face reverse spreads its state adjoints over its
+//                 stencil through the forward's minmod branches) -> lam_in of
+//                 rho, v, p and of the cell-averaged B
 //   k_adj_bfinish face B: cell-average and normal-field adjoints -> lam_in B
 #include "adjoint_hlld.h"
 #include "nmhd_kernels.h"
@@ -155,11 +156,25 @@ __global__ void __launch_bounds__(256) k_adj_prim(Box b, FieldPtrs Wout, FieldPt
 //
 // Face = lower D-face of cell (x,y,z).  Flux components in the forward's
 // order (rho, m_D, m_D+1, m_D+2, E, B_D+1, B_D+2).
+//
+// The face reverse hands the adjoints of its two reconstructed states to the
+// gather as their contributions to the four cells of the face's stencil:
+// c[m][k] is the adjoint reaching variable k of stencil cell c-2+m
+// (m = 0..3), so that the gather adds four values per variable instead of
+// recomputing the limiter branches from the stencils of four faces.
 struct AdjFaceOut {
-  float *aL[7];
-  float *aR[7];
+  float *c[4][7];
   float *abn;
 };
+
+// Derivative of minmod(a, b) with respect to a and b on the branch the
+// forward took (ties and sign changes as minmod takes them).
+__device__ __forceinline__ void mm_grad(float a, float b, float &da, float &db) {
+  const bool nz = !(a * b <= 0.0f);
+  const bool pick_a = fabsf(a) < fabsf(b);
+  da = (nz && pick_a) ? 1.0f : 0.0f;
+  db = (nz && !pick_a) ? 1.0f : 0.0f;
+}
 
 template <int D>
 __device__ __forceinline__ void adj_face(const Box &b, const FieldPtrs &W, const float *gU0,
@@ -266,9 +281,25 @@ __device__ __forceinline__ void adj_face(const Box &b, const FieldPtrs &W, const
   adj::hlld_reverse(b.gamma, b.inv_gm1, in, tape, aF, g);
   const float gl[7] = {g.rL, g.unL, g.u1L, g.u2L, g.pL, g.b1L, g.b2L};
   const float gr[7] = {g.rR, g.unR, g.u1R, g.u2R, g.pR, g.b1R, g.b2R};
+  // L = x1 + m(x1 - x0, x2 - x1)/2 and R = x2 - m(x2 - x1, x3 - x2)/2 over
+  // the stencil x0..x3 (cells c-2..c+1): their adjoints spread onto the
+  // four cells through the limiter branches the forward took
   for (int k = 0; k < 7; ++k) {
-    out.aL[k][a] = gl[k];
-    out.aR[k][a] = gr[k];
+    float c0 = 0.0f, c1 = gl[k], c2 = gr[k], c3 = 0.0f;
+    if (b.recon != 1) {
+      float da, db, da2, db2;
+      mm_grad(q[k][1] - q[k][0], q[k][2] - q[k][1], da, db);
+      mm_grad(q[k][2] - q[k][1], q[k][3] - q[k][2], da2, db2);
+      const float hl = 0.5f * gl[k], hr = 0.5f * gr[k];
+      c0 = -hl * da;
+      c1 += hl * (da - db) + hr * da2;
+      c2 += hl * db - hr * (da2 - db2);
+      c3 = -hr * db2;
+    }
+    out.c[0][k][a] = c0;
+    out.c[1][k][a] = c1;
+    out.c[2][k][a] = c2;
+    out.c[3][k][a] = c3;
   }
   out.abn[a] = g.bx;
 }
@@ -297,82 +328,37 @@ __global__ void __launch_bounds__(256) k_adj_faces(Box b, FieldPtrs W, const flo
 
 // ------------------------------------------------------------ k_adj_gather
 //
-// Cell c, variable x along D: L(face c+1) = x_c + s_c/2, R(face c) =
-// x_c - s_c/2 with the limited slope s_c = minmod(x_c - x_{c-1}, x_{c+1} - x_c).
-__device__ __forceinline__ void mm_grad(float a, float b, float &da, float &db) {
-  const bool nz = !(a * b <= 0.0f);
-  const bool pick_a = fabsf(a) < fabsf(b);
-  da = (nz && pick_a) ? 1.0f : 0.0f;
-  db = (nz && !pick_a) ? 1.0f : 0.0f;
-}
-
 // The reconstruction reverse of direction D at cell (x,y,z): res[0..6] =
-// adjoints of rho, v_D, v_D+1, v_D+2, p, Bbar_D+1, Bbar_D+2 of the cell.
+// adjoints of rho, v_D, v_D+1, v_D+2, p, Bbar_D+1, Bbar_D+2 of the cell, the
+// contributions of the four faces whose stencils hold it (faces c+2, c+1, c
+// and c-1 along D, where it is stencil cell 0, 1, 2 and 3).
 template <int D>
-__device__ __forceinline__ void adj_gather(const Box &b, const FieldPtrs &W, const AdjFaceOut &fa,
-                                           int x, int y, int z, float (&res)[7]) {
-  constexpr int D1 = (D + 1) % 3, D2 = (D + 2) % 3;
-  // cells c-2..c+2 and faces c-1..c+2 along D, with their coordinates
-  long cs[5];
-  int cc[5][3];
-  for (int s = 0; s < 5; ++s) {
-    cc[s][0] = x, cc[s][1] = y, cc[s][2] = z;
-    cc[s][D] = wrap(cc[s][D] + s - 2, b.n[D]);
-    cs[s] = cell_index(b, cc[s][0], cc[s][1], cc[s][2]);
-  }
-  // cell-centred B_t of stencil cell s: the average of its two t-faces
-  auto bavg = [&](int t, int s) -> float {
-    return 0.5f * (W.f[5 + t][cs[s]] + W.f[5 + t][nb(b, cc[s][0], cc[s][1], cc[s][2], t, 1)]);
-  };
-  for (int k = 0; k < 7; ++k) {
-    float xv[5];
-    for (int s = 0; s < 5; ++s) {
-      const long c = cs[s];
-      xv[s] = k == 0 ? W.f[0][c]
-            : k == 1 ? W.f[1 + D][c]
-            : k == 2 ? W.f[1 + D1][c]
-            : k == 3 ? W.f[1 + D2][c]
-            : k == 4 ? W.f[4][c]
-            : k == 5 ? bavg(D1, s)
-                     : bavg(D2, s);
-    }
-    // face f of cell cs[s] is its lower face; aL(face c+1) = fa.aL[k][cs[3]]
-    const float aL_c = fa.aL[k][cs[2]], aL_c1 = fa.aL[k][cs[3]], aL_c2 = fa.aL[k][cs[4]];
-    const float aR_cm1 = fa.aR[k][cs[1]], aR_c = fa.aR[k][cs[2]], aR_c1 = fa.aR[k][cs[3]];
-    float r = aL_c1 + aR_c;
-    if (b.recon != 1) {
-      // slope adjoints of cells c-1, c, c+1
-      const float as_m = 0.5f * (aL_c - aR_cm1);
-      const float as_0 = 0.5f * (aL_c1 - aR_c);
-      const float as_p = 0.5f * (aL_c2 - aR_c1);
-      float da_m, db_m, da_0, db_0, da_p, db_p;
-      mm_grad(xv[1] - xv[0], xv[2] - xv[1], da_m, db_m);
-      mm_grad(xv[2] - xv[1], xv[3] - xv[2], da_0, db_0);
-      mm_grad(xv[3] - xv[2], xv[4] - xv[3], da_p, db_p);
-      r += as_0 * (da_0 - db_0) - as_p * da_p + as_m * db_m;
-    }
-    res[k] = r;
-  }
+__device__ __forceinline__ void adj_gather(const Box &b, const AdjFaceOut &fa, int x, int y,
+                                           int z, float (&res)[7]) {
+  long f[4];
+  for (int m = 0; m < 4; ++m) f[m] = nb(b, x, y, z, D, 2 - m);
+  for (int k = 0; k < 7; ++k)
+    res[k] = fa.c[0][k][f[0]] + fa.c[1][k][f[1]] + fa.c[2][k][f[2]] + fa.c[3][k][f[3]];
 }
 
 // lam_in of rho, v, p from the three directions' reconstruction reverses
 // (written, or added to with accumulate), and the cell-average B adjoints
 // lbb[t] (written).
-__global__ void __launch_bounds__(256) k_adj_gather(Box b, FieldPtrs W, AdjFaceOut3 fa,
-                                                    FieldPtrs lam_in, int accumulate,
-                                                    float *lbb0, float *lbb1, float *lbb2) {
+__global__ void __launch_bounds__(256) k_adj_gather(Box b, AdjFaceOut3 fa, FieldPtrs lam_in,
+                                                    int accumulate, float *lbb0, float *lbb1,
+                                                    float *lbb2) {
   int x, y, z;
   long a;
   if (!cell_of(b, x, y, z, a)) return;
   float rho = 0.0f, v[3] = {0.0f, 0.0f, 0.0f}, p = 0.0f, bb[3] = {0.0f, 0.0f, 0.0f};
   float res[7];
-  adj_gather<0>(b, W, fa.d[0], x, y, z, res);  // X: (rho, vx, vy, vz, p, Bby, Bbz)
+  adj_gather<0>(b, fa.d[0], x, y, z, res);  // X: (rho, vx, vy, vz, p, Bby, Bbz)
   rho += res[0], v[0] += res[1], v[1] += res[2], v[2] += res[3], p += res[4];
   bb[1] += res[5], bb[2] += res[6];
-  adj_gather<1>(b, W, fa.d[1], x, y, z, res);  // Y: (rho, vy, vz, vx, p, Bbz, Bbx)
+  adj_gather<1>(b, fa.d[1], x, y, z, res);  // Y: (rho, vy, vz, vx, p, Bbz, Bbx)
   rho += res[0], v[1] += res[1], v[2] += res[2], v[0] += res[3], p += res[4];
   bb[2] += res[5], bb[0] += res[6];
-  adj_gather<2>(b, W, fa.d[2], x, y, z, res);  // Z: (rho, vz, vx, vy, p, Bbx, Bby)
+  adj_gather<2>(b, fa.d[2], x, y, z, res);  // Z: (rho, vz, vx, vy, p, Bbx, Bby)
   rho += res[0], v[2] += res[1], v[0] += res[2], v[1] += res[3], p += res[4];
   bb[0] += res[5], bb[1] += res[6];
   const float out[5] = {rho, v[0], v[1], v[2], p};
@@ -429,17 +415,15 @@ void launch_adj_stage(hipStream_t s, const Box &b, const AdjStageArgs &A) {
                        A.lam_base);
   AdjFaceOut3 fo;
   for (int d = 0; d < 3; ++d) {
-    for (int k = 0; k < 7; ++k) {
-      fo.d[d].aL[k] = A.face[d * 14 + k];
-      fo.d[d].aR[k] = A.face[d * 14 + 7 + k];
-    }
+    for (int m = 0; m < 4; ++m)
+      for (int k = 0; k < 7; ++k) fo.d[d].c[m][k] = A.face[d * 28 + m * 7 + k];
     fo.d[d].abn = A.abn[d];
   }
   const dim3 g3(grid_for(b.ncell), 3);
   hipLaunchKernelGGL(k_adj_faces, g3, blk, 0, s, b, A.Win, A.gU[0], A.gU[1], A.gU[2], A.gU[3],
                      A.gU[4], A.lE[0], A.lE[1], A.lE[2], A.h, fo);
-  hipLaunchKernelGGL(k_adj_gather, g, blk, 0, s, b, A.Win, fo, A.lam_in, A.acc_in ? 1 : 0,
-                     A.lbb[0], A.lbb[1], A.lbb[2]);
+  hipLaunchKernelGGL(k_adj_gather, g, blk, 0, s, b, fo, A.lam_in, A.acc_in ? 1 : 0, A.lbb[0],
+                     A.lbb[1], A.lbb[2]);
   hipLaunchKernelGGL(k_adj_bfinish, g, blk, 0, s, b, A.lbb[0], A.lbb[1], A.lbb[2], A.abn[0],
                      A.abn[1], A.abn[2], A.lam_in, A.acc_in ? 1 : 0);
 }

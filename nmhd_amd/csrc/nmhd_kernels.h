@@ -202,8 +202,9 @@ struct AdjStageArgs {
   FieldPtrs lam_in;            // dL/dW_in through P (written, or added to with acc_in)
   float h;
   bool acc_base, acc_in;
-  // work arrays (ncell floats each); face: 14 per direction (aL[7], aR[7])
-  float *gU[5], *lE[3], *lbb[3], *abn[3], *face[42];
+  // work arrays (ncell floats each); face: 28 per direction, the adjoints
+  // of a face's states spread over its 4 stencil cells (4 x 7 variables)
+  float *gU[5], *lE[3], *lbb[3], *abn[3], *face[84];
 };
 void launch_adj_set_dt(hipStream_t s, StepState *st, float dt);
 void launch_adj_stage(hipStream_t s, const Box &b, const AdjStageArgs &A);
