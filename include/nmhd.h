@@ -299,7 +299,7 @@ int nmhd_adj_get_geometry(const nmhd_adj *adj, nmhd_geometry *geo);
 /* Launch later work on `stream`, a hipStream_t such as
  * torch.cuda.current_stream().cuda_stream; NULL selects HIP's null stream
  * (torch's default stream).  Until the first call the object uses a stream
- * of its own. */
+ * of its own, which the first call releases. */
 int nmhd_adj_set_stream(nmhd_adj *adj, void *stream);
 int nmhd_adj_synchronize(nmhd_adj *adj);
 /* The CFL dt of a device state (as nmhd_cfl_dt).  Synchronises. */

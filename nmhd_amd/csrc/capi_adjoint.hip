@@ -37,7 +37,7 @@ void enqueue_frozen_step(hipStream_t s, const Box &b, int integrator, FieldPtrs 
 namespace {
 // per thread: the message of the calling thread's last object-less failure
 thread_local std::string g_adj_create_error;
-constexpr int kWork = 5 + 3 + 3 + 3 + 84;  // gU, lE, lbb, abn, face
+constexpr int kWork = 5 + 3 + 3 + 3 + 45;  // gU, lE, lbb, abn, face
 }  // namespace
 
 struct nmhd_adj {
@@ -106,7 +106,7 @@ AdjStageArgs stage_args(nmhd_adj *a) {
   for (int i = 0; i < 3; ++i) A.lE[i] = *w++;
   for (int i = 0; i < 3; ++i) A.lbb[i] = *w++;
   for (int i = 0; i < 3; ++i) A.abn[i] = *w++;
-  for (int i = 0; i < 84; ++i) A.face[i] = *w++;
+  for (int i = 0; i < 45; ++i) A.face[i] = *w++;
   return A;
 }
 
@@ -199,6 +199,13 @@ int nmhd_adj_set_stream(nmhd_adj *a, void *stream) {
   if (!a) return NMHD_EINVAL;
   if (a->stream != (hipStream_t)stream) AHIP(a, hipStreamSynchronize(a->stream));
   a->stream = (hipStream_t)stream;  // NULL: HIP's null stream
+  // the object's own stream is used only until the caller names one: release
+  // it, so that it does not share the process's few hardware queues with
+  // the caller's streams (streams beyond them share a queue and serialise)
+  if (a->own && a->own != a->stream) {
+    AHIP(a, hipStreamDestroy(a->own));
+    a->own = nullptr;
+  }
   return NMHD_OK;
 }
 

@@ -19,10 +19,10 @@
 //                 (adjoint_hlld.h) -> adjoints of the 7 left/right face states
 //                 and of the normal field
 //   k_adj_gather  per cell: the reconstruction adjoints of the three
-//                 directions, summed from the 4 faces whose stencils hold the
-//                 cell (each face reverse spreads its state adjoints over its
-//                 stencil through the forward's minmod branches) -> lam_in of
-//                 rho, v, p and of the cell-averaged B
+//                 directions, gathered from the 4 faces whose stencils hold
+//                 the cell (their state adjoints and the minmod branches the
+//                 face reverse recorded) -> lam_in of rho, v, p and of the
+//                 cell-averaged B
 //   k_adj_bfinish face B: cell-average and normal-field adjoints -> lam_in B
 #include "adjoint_hlld.h"
 #include "nmhd_kernels.h"
@@ -157,13 +157,17 @@ __global__ void __launch_bounds__(256) k_adj_prim(Box b, FieldPtrs Wout, FieldPt
 // Face = lower D-face of cell (x,y,z).  Flux components in the forward's
 // order (rho, m_D, m_D+1, m_D+2, E, B_D+1, B_D+2).
 //
-// The face reverse hands the adjoints of its two reconstructed states to the
-// gather as their contributions to the four cells of the face's stencil:
-// c[m][k] is the adjoint reaching variable k of stencil cell c-2+m
-// (m = 0..3), so that the gather adds four values per variable instead of
-// recomputing the limiter branches from the stencils of four faces.
+// The face reverse hands the gather the adjoints of its two reconstructed
+// states, aL[k] and aR[k] (variable k), and the limiter branches the
+// forward took for them, 4 bits per variable in one word: bit 4k (4k+1)
+// when the left state's limited slope took its lower (upper) difference,
+// bit 4k+2 (4k+3) likewise for the right state's; no bit for a zero slope.
+// The gather then spreads the adjoints over the stencil cells without
+// reading the stencils again.
 struct AdjFaceOut {
-  float *c[4][7];
+  float *aL[7];
+  float *aR[7];
+  unsigned *br;
   float *abn;
 };
 
@@ -269,6 +273,19 @@ __device__ __forceinline__ void adj_face(const Box &b, const FieldPtrs &W, const
       recon_minbee2_med3(q[k][0], q[k][1], q[k][2], q[k][3], Lq[k], Rq[k]);
     }
   }
+  // the limiter branches of L = x1 + m(x1 - x0, x2 - x1)/2 and
+  // R = x2 - m(x2 - x1, x3 - x2)/2 over the stencil x0..x3 (cells c-2..c+1),
+  // for the gather
+  unsigned br = 0u;
+  if (b.recon != 1) {
+    for (int k = 0; k < 7; ++k) {
+      float da, db, da2, db2;
+      mm_grad(q[k][1] - q[k][0], q[k][2] - q[k][1], da, db);
+      mm_grad(q[k][2] - q[k][1], q[k][3] - q[k][2], da2, db2);
+      br |= ((da != 0.0f ? 1u : 0u) | (db != 0.0f ? 2u : 0u) | (da2 != 0.0f ? 4u : 0u) |
+             (db2 != 0.0f ? 8u : 0u)) << (4 * k);
+    }
+  }
   adj::HIn<float> in;
   in.bx = W.f[5 + D][a];
   in.rL = Lq[0], in.rR = Rq[0], in.unL = Lq[1], in.unR = Rq[1], in.u1L = Lq[2], in.u1R = Rq[2];
@@ -281,26 +298,11 @@ __device__ __forceinline__ void adj_face(const Box &b, const FieldPtrs &W, const
   adj::hlld_reverse(b.gamma, b.inv_gm1, in, tape, aF, g);
   const float gl[7] = {g.rL, g.unL, g.u1L, g.u2L, g.pL, g.b1L, g.b2L};
   const float gr[7] = {g.rR, g.unR, g.u1R, g.u2R, g.pR, g.b1R, g.b2R};
-  // L = x1 + m(x1 - x0, x2 - x1)/2 and R = x2 - m(x2 - x1, x3 - x2)/2 over
-  // the stencil x0..x3 (cells c-2..c+1): their adjoints spread onto the
-  // four cells through the limiter branches the forward took
   for (int k = 0; k < 7; ++k) {
-    float c0 = 0.0f, c1 = gl[k], c2 = gr[k], c3 = 0.0f;
-    if (b.recon != 1) {
-      float da, db, da2, db2;
-      mm_grad(q[k][1] - q[k][0], q[k][2] - q[k][1], da, db);
-      mm_grad(q[k][2] - q[k][1], q[k][3] - q[k][2], da2, db2);
-      const float hl = 0.5f * gl[k], hr = 0.5f * gr[k];
-      c0 = -hl * da;
-      c1 += hl * (da - db) + hr * da2;
-      c2 += hl * db - hr * (da2 - db2);
-      c3 = -hr * db2;
-    }
-    out.c[0][k][a] = c0;
-    out.c[1][k][a] = c1;
-    out.c[2][k][a] = c2;
-    out.c[3][k][a] = c3;
+    out.aL[k][a] = gl[k];
+    out.aR[k][a] = gr[k];
   }
+  out.br[a] = br;
   out.abn[a] = g.bx;
 }
 
@@ -329,16 +331,34 @@ __global__ void __launch_bounds__(256) k_adj_faces(Box b, FieldPtrs W, const flo
 // ------------------------------------------------------------ k_adj_gather
 //
 // The reconstruction reverse of direction D at cell (x,y,z): res[0..6] =
-// adjoints of rho, v_D, v_D+1, v_D+2, p, Bbar_D+1, Bbar_D+2 of the cell, the
-// contributions of the four faces whose stencils hold it (faces c+2, c+1, c
-// and c-1 along D, where it is stencil cell 0, 1, 2 and 3).
+// adjoints of rho, v_D, v_D+1, v_D+2, p, Bbar_D+1, Bbar_D+2 of the cell,
+// gathered from the four faces whose stencils hold it: faces c+2, c+1, c
+// and c-1 along D, where it is stencil cell 0, 1, 2 and 3.  A face's left
+// state L = x1 + m/2 and right state R = x2 - m'/2 (m the limited slope of
+// x0, x1, x2, m' that of x1, x2, x3) pass their adjoints aL, aR to the
+// stencil cells through the branches the forward took:
+//   cell 0: -aL/2 [m took x1 - x0]
+//   cell 1:  aL + aL/2 ([m took x1 - x0] - [m took x2 - x1]) + aR/2 [m' took x2 - x1]
+//   cell 2:  aR + aL/2 [m took x2 - x1] - aR/2 ([m' took x2 - x1] - [m' took x3 - x2])
+//   cell 3: -aR/2 [m' took x3 - x2]
 template <int D>
 __device__ __forceinline__ void adj_gather(const Box &b, const AdjFaceOut &fa, int x, int y,
                                            int z, float (&res)[7]) {
-  long f[4];
-  for (int m = 0; m < 4; ++m) f[m] = nb(b, x, y, z, D, 2 - m);
-  for (int k = 0; k < 7; ++k)
-    res[k] = fa.c[0][k][f[0]] + fa.c[1][k][f[1]] + fa.c[2][k][f[2]] + fa.c[3][k][f[3]];
+  const long f0 = nb(b, x, y, z, D, 2), f1 = nb(b, x, y, z, D, 1), f3 = nb(b, x, y, z, D, -1);
+  const long f2 = cell_index(b, x, y, z);
+  const unsigned b0 = fa.br[f0], b1 = fa.br[f1], b2 = fa.br[f2], b3 = fa.br[f3];
+  auto bit = [](unsigned w, int i) { return ((w >> i) & 1u) ? 1.0f : 0.0f; };
+  for (int k = 0; k < 7; ++k) {
+    const int s = 4 * k;
+    const float l0 = fa.aL[k][f0], l1 = fa.aL[k][f1], r1 = fa.aR[k][f1];
+    const float l2 = fa.aL[k][f2], r2 = fa.aR[k][f2], r3 = fa.aR[k][f3];
+    const float c0 = -0.5f * l0 * bit(b0, s);
+    const float c1 = l1 + 0.5f * l1 * (bit(b1, s) - bit(b1, s + 1)) + 0.5f * r1 * bit(b1, s + 2);
+    const float c2 =
+        r2 + 0.5f * l2 * bit(b2, s + 1) - 0.5f * r2 * (bit(b2, s + 2) - bit(b2, s + 3));
+    const float c3 = -0.5f * r3 * bit(b3, s + 3);
+    res[k] = c0 + c1 + c2 + c3;
+  }
 }
 
 // lam_in of rho, v, p from the three directions' reconstruction reverses
@@ -415,8 +435,11 @@ void launch_adj_stage(hipStream_t s, const Box &b, const AdjStageArgs &A) {
                        A.lam_base);
   AdjFaceOut3 fo;
   for (int d = 0; d < 3; ++d) {
-    for (int m = 0; m < 4; ++m)
-      for (int k = 0; k < 7; ++k) fo.d[d].c[m][k] = A.face[d * 28 + m * 7 + k];
+    for (int k = 0; k < 7; ++k) {
+      fo.d[d].aL[k] = A.face[d * 15 + k];
+      fo.d[d].aR[k] = A.face[d * 15 + 7 + k];
+    }
+    fo.d[d].br = reinterpret_cast<unsigned *>(A.face[d * 15 + 14]);
     fo.d[d].abn = A.abn[d];
   }
   const dim3 g3(grid_for(b.ncell), 3);

@@ -202,9 +202,10 @@ struct AdjStageArgs {
   FieldPtrs lam_in;            // dL/dW_in through P (written, or added to with acc_in)
   float h;
   bool acc_base, acc_in;
-  // work arrays (ncell floats each); face: 28 per direction, the adjoints
-  // of a face's states spread over its 4 stencil cells (4 x 7 variables)
-  float *gU[5], *lE[3], *lbb[3], *abn[3], *face[84];
+  // work arrays (ncell floats each); face: 15 per direction, the adjoints
+  // of a face's left and right states (7 + 7) and its limiter branches
+  // (one 32-bit word per face)
+  float *gU[5], *lE[3], *lbb[3], *abn[3], *face[45];
 };
 void launch_adj_set_dt(hipStream_t s, StepState *st, float dt);
 void launch_adj_stage(hipStream_t s, const Box &b, const AdjStageArgs &A);
