@@ -56,6 +56,10 @@ def build_cpp():
     _run(flags + inc + [os.path.join(HERE, "apps", "kepler_disk.cpp"),
                         "-o", os.path.join(bindir, "kepler_disk")] + link +
          ["-Wl,-rpath,$ORIGIN/.."])
+    # the VALU issue-rate probe that places the face kernel on its roofline
+    _run(["hipcc", "--offload-arch=gfx950", "-O3", "-Xclang", "-target-feature", "-Xclang",
+          "-packed-fp32-ops", os.path.join(HERE, "apps", "valu_rate.hip"),
+          "-o", os.path.join(bindir, "valu_rate")])
     # host (double) build of the HLLD reverse sweep for the CPU derivative tests
     _run(["hipcc", "-x", "hip", "--offload-host-only", "-O2", "-std=c++17", "-fPIC", "-shared",
           "-I", CSRC, os.path.join(ROOT, "tests", "cpp", "adj_hlld_host.cpp"),
