@@ -55,3 +55,34 @@ def test_noalloc_rejects_boxes_beyond_the_scratch(gpu_ready):
     assert not out.any()  # nothing was launched
     with pytest.raises(capi.NmhdError, match="exceeds the no-allocation scratch"):
         A.forward_noalloc(big, W, 1e-4, out)
+
+
+def test_noalloc_step_replayed_from_a_captured_graph(gpu_ready):
+    """The use the entry point is for: a step captured into a HIP graph (a
+    torch CUDA graph here) after the first call on the device has done the
+    set-up outside the capture.  Each replay, with the program's own eager
+    work and a new input between replays, equals the allocating step on the
+    same input bit for bit."""
+    print("\n[case] noalloc step captured in a graph, 3 replays", flush=True)
+    cfg, W0 = _case((32, 24, 16), 0, 4)
+    S = A.Stepper(cfg)
+    dt = S.cfl_dt(W0)
+    A.forward_noalloc(cfg, W0, dt, torch.empty_like(W0))  # set-up outside the capture
+    torch.cuda.synchronize()
+    static_in = W0.clone()
+    static_out = torch.zeros_like(W0)
+    graph = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(graph):
+        A.forward_noalloc(cfg, static_in, dt, static_out)
+    torch.cuda.synchronize()
+    for rep in range(3):
+        _, Wr = _case((32, 24, 16), 0, 10 + rep)
+        ref = S.forward(Wr, dt)
+        static_in.copy_(Wr)
+        static_out.fill_(float("nan"))
+        torch.cuda.synchronize()
+        graph.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(static_out.cpu().numpy().view(np.uint32),
+                              ref.cpu().numpy().view(np.uint32)), rep
+    S.close()
