@@ -229,20 +229,28 @@ void update_dispatch(nmhd_ctx *c, FieldPtrs Win, const FluxSet &F, FieldPtrs Wb,
 }
 
 // One iteration of the proceed() loop (INL:1094-1201), enqueued.
-void enqueue_step(nmhd_ctx *c, float goal) {
+// after_step: the previous step of the same enqueue call left its commit to
+// this step's opening launch; defer_commit: leave this step's commit to the
+// next one (steps enqueued back to back share one bookkeeping launch).
+void enqueue_step(nmhd_ctx *c, float goal, bool after_step = false, bool defer_commit = false) {
   const Box &b = c->box;
   hipStream_t s = c->stream;
   const int A = c->cur;
   const bool tiled = c->path == NMHD_PATH_TILED;
   const bool full_cfl = !(tiled && c->cfl_valid);
-  timed(c, S_BEGIN, [&] { launch_begin_step(s, c->st, full_cfl); });
+  const bool ghosts = b.m[1] > 0 || b.m[2] > 0;
+  // with the carried CFL minimum and no ghost cells to add to it, dt is
+  // decided in the opening launch
+  const bool dt_in_open = !full_cfl && !ghosts;
+  timed(c, S_BEGIN,
+        [&] { launch_open_step(s, c->st, after_step, full_cfl, dt_in_open, goal); });
   if (b.m[1] > 0) timed(c, S_GHOST_Y, [&] { launch_ghost_y(s, b, fields(c, A), c->st); });
   if (b.m[2] > 0) timed(c, S_GHOST_Z, [&] { launch_ghost_z(s, b, fields(c, A), c->st); });
   if (full_cfl)
     timed(c, S_CFL, [&] { launch_cfl(s, b, fields(c, A), goal, c->st); });
-  else
+  else if (ghosts)
     timed(c, S_CFL_GHOSTS, [&] { launch_cfl_ghosts(s, b, fields(c, A), c->st); });
-  timed(c, S_DT, [&] { launch_dt(s, c->st, goal); });
+  if (!dt_in_open) timed(c, S_DT, [&] { launch_dt(s, c->st, goal); });
   const FluxSet F = fluxes(c);
   const int slots[3] = {S_FLUX_X, S_FLUX_Y, S_FLUX_Z};
   // one RK stage: fluxes of W[in], update of the conserved base W[base] into W[out]
@@ -272,8 +280,11 @@ void enqueue_step(nmhd_ctx *c, float goal) {
     stage(A, A, B, H_HALF, false);
     stage(B, A, next, H_FULL, true);
   }
-  timed(c, S_COPY, [&] { launch_copy_state(s, b, fields(c, A), fields(c, next), c->st); });
-  timed(c, S_COMMIT, [&] { launch_commit(s, c->st); });
+  // an inactive step carries the state into the next buffer: on the tiled
+  // path the last stage's update kernel does it, the other paths copy
+  if (!tiled)
+    timed(c, S_COPY, [&] { launch_copy_state(s, b, fields(c, A), fields(c, next), c->st); });
+  if (!defer_commit) timed(c, S_COMMIT, [&] { launch_commit(s, c->st); });
   c->cur = next;
   c->cfl_valid = tiled;
 }
@@ -508,7 +519,7 @@ int nmhd_reset_run(nmhd_ctx *c) {
 
 int nmhd_enqueue_steps(nmhd_ctx *c, float goal_time, int nsteps) {
   if (!c || nsteps < 0) return NMHD_EINVAL;
-  for (int k = 0; k < nsteps; ++k) enqueue_step(c, goal_time);
+  for (int k = 0; k < nsteps; ++k) enqueue_step(c, goal_time, k > 0, k + 1 < nsteps);
   HIPCHECK(c, hipGetLastError());
   return NMHD_OK;
 }

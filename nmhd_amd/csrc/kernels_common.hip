@@ -16,6 +16,15 @@ namespace nmhd {
 
 __global__ void k_begin_step(StepState *st, int full) { step_begin(*st, full); }
 
+// The bookkeeping between two steps enqueued back to back, in one launch:
+// commit the previous step (when commit_prev), begin the next one and, when
+// its CFL minimum needs no pass over cells (dt_too), decide its dt.
+__global__ void k_open_step(StepState *st, int commit_prev, int full, int dt_too, float goal) {
+  if (commit_prev) step_commit(*st);
+  step_begin(*st, full);
+  if (dt_too) step_dt(*st, goal);
+}
+
 // relative_displacement (INL:288-295) with D0 = 0 and
 // relvel = -q*Omega*kLocalExtentY (application/kepler_disk_demo/
 // mhd_solver.curb:46-52), in FP64 as the reference evaluates it.
@@ -218,6 +227,11 @@ __global__ void k_diag_prepare(StepState *diag, const StepState *st, float h) {
 
 void launch_begin_step(hipStream_t s, StepState *st, bool full) {
   hipLaunchKernelGGL(k_begin_step, dim3(1), dim3(1), 0, s, st, (int)full);
+}
+void launch_open_step(hipStream_t s, StepState *st, bool commit_prev, bool full, bool dt_too,
+                      float goal) {
+  hipLaunchKernelGGL(k_open_step, dim3(1), dim3(1), 0, s, st, (int)commit_prev, (int)full,
+                     (int)dt_too, goal);
 }
 // Blocks of a reduction kernel: one per 1024 cells, at most 512.
 unsigned reduce_grid(long n) {

@@ -32,11 +32,24 @@ __global__ void __launch_bounds__(kTileThreads) k_update_march(Box b, FieldPtrs 
                                                                FieldPtrs Wb, FieldPtrs Wo,
                                                                StepState *st, int hsel, int zc,
                                                                int gx, int gy) {
-  if (!st->active) return;
-  __shared__ float lds[UpdateShape<TX, TY>::kLds];
-  __shared__ float red[4];
   const int lb = xcd_logical_block(blockIdx.x, gridDim.x);
   const int x0 = (lb % gx) * TX, y0 = ((lb / gx) % gy) * TY, z0 = (lb / gx / gy) * zc;
+  if (!st->active) {
+    // a step enqueued after the run reached its goal: the last stage's
+    // output carries the step's base state through the buffer rotation
+    if (Cfl && !PaceOnly) {
+      const int x = x0 + (int)threadIdx.x % TX, y = y0 + (int)threadIdx.x / TX;
+      const int zend = min(z0 + zc, b.n[2]);
+      if (x < b.n[0] && y < b.n[1])
+        for (int r = z0; r < zend; ++r) {
+          const long a = ((long)r * b.n[1] + y) * b.n[0] + x;
+          for (int f = 0; f < 8; ++f) Wo.f[f][a] = Wb.f[f][a];
+        }
+    }
+    return;
+  }
+  __shared__ float lds[UpdateShape<TX, TY>::kLds];
+  __shared__ float red[4];
   float cfl_t = 3.0e38f;
   update_tile<TX, TY, PaceOnly, Cfl>(b, Win, F, Wb, Wo, stage_h(st, hsel), x0, y0, z0,
                                      min(z0 + zc, b.n[2]), lds, cfl_t);

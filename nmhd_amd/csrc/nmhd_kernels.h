@@ -141,6 +141,10 @@ constexpr int kTileTY[TV_COUNT] = {4, 8, 16};
 
 // kernels_common.hip
 void launch_begin_step(hipStream_t s, StepState *st, bool full);
+// commit the previous step (commit_prev), begin the next (full CFL pass or
+// the carried minimum) and, with dt_too, decide its dt, in one launch
+void launch_open_step(hipStream_t s, StepState *st, bool commit_prev, bool full, bool dt_too,
+                      float goal);
 void launch_cfl_ghosts(hipStream_t s, const Box &b, FieldPtrs W, StepState *st);
 void launch_ghost_y(hipStream_t s, const Box &b, FieldPtrs W, const StepState *st);
 void launch_ghost_z(hipStream_t s, const Box &b, FieldPtrs W, const StepState *st);
