@@ -73,6 +73,26 @@ enum nmhd_path {
   NMHD_PATH_STRICT = 3
 };
 
+/* Corrections of reference behaviour (nmhd_desc.fixes, a bit set).  The
+ * default, 0, reproduces the reference exactly, quirks included; each bit
+ * replaces one quirk by what the reference's code evidently intends. */
+enum nmhd_fix {
+  /* CFL signal speed |v_d| + c instead of |v_d + c| (INL:715-720), which
+   * underestimates the speed where v_d < 0 */
+  NMHD_FIX_CFL_SPEED = 1,
+  /* shearing-periodic fill interpolates linearly between the two source
+   * cells by the fractional displacement, with the weights the gather
+   * kernel computes (INL:100-110: val_inc1 / val_dec1, read but unused),
+   * instead of copying the cell at the floor of the displacement */
+  NMHD_FIX_SHEAR_INTERP = 2,
+  /* face predictor's tidal term 2 q Omega^2 Y at the radial face's own
+   * position instead of the centre of its cell, half a cell further out
+   * (`Bit & 2 == 0` parses as `Bit & (2 == 0)`, crystal.h:86-104, used by
+   * INL:793-797) */
+  NMHD_FIX_FACE_POSITION = 4,
+  NMHD_FIX_ALL = 7
+};
+
 /* Tile shapes (TX x TY columns marched along z) of the tiled kernels. */
 enum nmhd_tile {
   NMHD_TILE_64x4 = 0,
@@ -97,6 +117,7 @@ typedef struct nmhd_desc {
   int recon;               /* enum nmhd_recon                                   */
   int integrator;          /* enum nmhd_integrator                              */
   int path;                /* enum nmhd_path                                    */
+  int fixes;               /* enum nmhd_fix bits; 0 = the reference's behaviour */
 } nmhd_desc;
 
 /* Derived geometry (LocalSetup members, mhd_solver.hrb:93-101). */

@@ -51,6 +51,7 @@ struct SimulationConfig {
   int recon = NMHD_RECON_MINBEE2;         // SpaceInterpolation
   int integrator = NMHD_MIDPOINT2;        // TimeIntegral
   int path = NMHD_PATH_AUTO;              // kernel path
+  int fixes = 0;                          // enum nmhd_fix bits (0: as the reference)
 };
 
 inline PreciseReal wallclock_now() {  // get_time.h
@@ -293,7 +294,7 @@ class MHDSolver {
 
   explicit MHDSolver(const SimulationConfig &cfg, LocalSetup &local_setup = default_local())
       : cfg_(cfg), local_setup_(local_setup) {
-    nmhd_desc d;
+    nmhd_desc d{};
     for (int i = 0; i < 3; ++i) {
       d.global_size[i] = cfg.global_size[i];
       d.margin[i] = cfg.margin[i];
@@ -308,6 +309,7 @@ class MHDSolver {
     d.recon = cfg.recon;
     d.integrator = cfg.integrator;
     d.path = cfg.path;
+    d.fixes = cfg.fixes;
     if (nmhd_create(&d, &ctx_) != NMHD_OK)
       throw std::runtime_error(std::string("nmhd_create: ") + nmhd_last_error(0));
     nmhd_get_geometry(ctx_, &geo_);

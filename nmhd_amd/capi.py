@@ -36,6 +36,8 @@ EXPORTS = (
     "nmhd_adj_forward_noalloc",
 )
 TILE_64x4, TILE_32x8, TILE_16x16 = 0, 1, 2
+# enum nmhd_fix: corrections of reference quirks (0 = the reference's behaviour)
+FIX_CFL_SPEED, FIX_SHEAR_INTERP, FIX_FACE_POSITION, FIX_ALL = 1, 2, 4, 7
 
 
 class NmhdDesc(ctypes.Structure):
@@ -52,6 +54,7 @@ class NmhdDesc(ctypes.Structure):
         ("recon", ctypes.c_int),
         ("integrator", ctypes.c_int),
         ("path", ctypes.c_int),
+        ("fixes", ctypes.c_int),
     ]
 
 
@@ -178,6 +181,7 @@ def desc_from_config(cfg, path=PATH_AUTO):
     d.recon = cfg.get("recon", 0)
     d.integrator = cfg.get("integrator", 0)
     d.path = path
+    d.fixes = cfg.get("fixes", 0)
     return d
 
 

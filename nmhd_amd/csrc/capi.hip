@@ -42,6 +42,7 @@ bool desc_valid(const nmhd_desc *d, std::string &why) {
   if (d->integrator != NMHD_MIDPOINT2 && d->integrator != NMHD_EULER1)
     return why = "unknown integrator", false;
   if (d->path < NMHD_PATH_AUTO || d->path > NMHD_PATH_STRICT) return why = "unknown path", false;
+  if (d->fixes & ~NMHD_FIX_ALL) return why = "unknown bit in fixes", false;
   long n = 1;
   for (int i = 0; i < 3; ++i) n *= (long)(d->global_size[i] + 2 * d->margin[i]);
   if (n >= (1L << 31)) return why = "box exceeds 2^31 cells", false;
@@ -79,6 +80,9 @@ void derive_box(const nmhd_desc *d, Box &b, nmhd_geometry &g) {
   b.qshear = d->qshear;
   b.shear_y = d->shear_y;
   b.recon = d->recon;
+  b.fix_cfl = (d->fixes & NMHD_FIX_CFL_SPEED) != 0;
+  b.fix_interp = (d->fixes & NMHD_FIX_SHEAR_INTERP) != 0;
+  b.face_off = (d->fixes & NMHD_FIX_FACE_POSITION) ? 0.0f : 0.5f;
 }
 
 }  // namespace nmhd

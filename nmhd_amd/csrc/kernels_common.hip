@@ -46,7 +46,9 @@ void host_displacement(const Box &b, double t, float *normalized, int *n, float 
 // One thread per (x, row, z) of the M-row ghost slab: the lower ghost row j
 // takes the top interior row j shifted by +n along X, the upper ghost row
 // takes the bottom interior row shifted by -n-1, v_X offset by -/+ relvel;
-// the first upper ghost By face (the outer radial face) is kept.
+// the first upper ghost By face (the outer radial face) is kept.  With
+// fix_interp the shifted values are interpolated between the cells at +n
+// and +n+1 (-n-1 and -n) with the gather kernel's weights (INL:100-110).
 __global__ void k_ghost_y(Box b, FieldPtrs W, const StepState *st) {
 #pragma clang fp contract(off)
   if (st->finished) return;
@@ -61,25 +63,34 @@ __global__ void k_ghost_y(Box b, FieldPtrs W, const StepState *st) {
   const int sy = (int)(row - (unsigned)sz * (unsigned)M);
   int n = 0;
   float dr = 0.0f, nd, relvel = 0.0f;
-  int ix = sx, dx = sx;
+  int ix = sx, dx = sx, ix1 = sx, dx1 = sx;
   if (b.shear_y) {
     displacement(b, st->t, &nd, &n, &dr);
     relvel = -b.qshear * b.omega * b.local_ext[1];
     n %= nx;  // same shift, and no int overflow in sx + n for huge t
     ix = ((sx + n) % nx + nx) % nx;
     dx = ((sx - n - 1) % nx + nx) % nx;
+    ix1 = ix + 1 < nx ? ix + 1 : 0;
+    dx1 = dx + 1 < nx ? dx + 1 : 0;
   }
   const long src_inc = addr(b, ix, ny - 2 * M + sy, sz);
   const long src_dec = addr(b, dx, M + sy, sz);
+  const long src_inc1 = addr(b, ix1, ny - 2 * M + sy, sz);
+  const long src_dec1 = addr(b, dx1, M + sy, sz);
   const long dst_inc = addr(b, sx, sy, sz);
   const long dst_dec = addr(b, sx, ny - M + sy, sz);
+  const bool interp = b.shear_y && b.fix_interp;
   for (int f = 0; f < 8; ++f) {
     float vi = W.f[f][src_inc];
     float vd = W.f[f][src_dec];
+    // the second source cells (the reference's val_inc1 / val_dec1)
+    const float vi1 = interp ? W.f[f][src_inc1] : vi;
+    const float vd1 = interp ? W.f[f][src_dec1] : vd;
     if (b.shear_y) {
-      // (1-d)*v + d*v, exactly as the gather kernel writes it (INL:108-110)
-      float bi = (1.0f - dr) * vi + dr * vi;
-      float bd = dr * vd + (1.0f - dr) * vd;
+      // (1-d)*v + d*v, exactly as the gather kernel writes it (INL:108-110);
+      // with fix_interp (1-d)*v0 + d*v1 and d*v0 + (1-d)*v1
+      float bi = (1.0f - dr) * vi + dr * vi1;
+      float bd = dr * vd + (1.0f - dr) * vd1;
       if (f == 1) {
         bi = bi - relvel;
         bd = bd + relvel;

@@ -32,6 +32,10 @@ struct Box {
   float omega, qshear;
   int shear_y;
   int recon;        // 0 minbee2, 1 piecewiseconst1
+  int fix_cfl;      // CFL speed |v| + c instead of |v + c|
+  int fix_interp;   // shear fill interpolates between the two source cells
+  float face_off;   // radial face position in the predictor: y + face_off
+                    // cells (0.5: the reference's cell centre; 0: the face)
 };
 
 // Device-side step record: dt is computed, clipped and consumed on the GPU so
@@ -366,11 +370,15 @@ __device__ __forceinline__ Flux7 hlld_strict(float gamma, float Bx, float rL, fl
   return F;
 }
 
-// Radial (Y) position of row y of the array relative to the box centre,
-// corner + d*(y + 0.5) with corner = -ext/2.  Every staggered position gets
-// the +0.5 (crystal.h:86-104), so faces use the centre of their own index.
+// Radial (Y) position y + off cells from the array's lower edge, relative
+// to the box centre: corner + d*(y + off) with corner = -ext/2.
+__device__ __forceinline__ float radial_position_at(const Box &b, int y, float off) {
+  return -b.ext[1] * 0.5f + b.d[1] * ((float)y + off);
+}
+// Centre of row y.  Every staggered position gets the same +0.5
+// (crystal.h:86-104), so the reference evaluates faces there too.
 __device__ __forceinline__ float radial_position(const Box &b, int y) {
-  return -b.ext[1] * 0.5f + b.d[1] * ((float)y + 0.5f);
+  return radial_position_at(b, y, 0.5f);
 }
 
 // Shearing-box acceleration (KEP:78-80): a_X = -2 Omega v_Y,
@@ -380,6 +388,18 @@ __device__ __forceinline__ float accel_x(const Box &b, float vY) {
 }
 __device__ __forceinline__ float accel_y(const Box &b, float vX, int y) {
   return 2.0f * b.omega * vX + 2.0f * b.qshear * (b.omega * b.omega) * radial_position(b, y);
+}
+// The same on the radial face below cell y, for the face predictor: at
+// y + 0.5 cells as the reference evaluates it, or at the face itself
+// (face_off = 0, NMHD_FIX_FACE_POSITION).
+__device__ __forceinline__ float accel_y_face(const Box &b, float vX, int y) {
+  return 2.0f * b.omega * vX +
+         2.0f * b.qshear * (b.omega * b.omega) * radial_position_at(b, y, b.face_off);
+}
+// CFL signal speed along one axis: |v + c| as the reference computes it
+// (INL:715-720), or |v| + c (NMHD_FIX_CFL_SPEED; c > 0).
+__device__ __forceinline__ float cfl_speed(const Box &b, float v, float c) {
+  return fabsf((b.fix_cfl ? fabsf(v) : v) + c);
 }
 
 }  // namespace nmhd

@@ -99,7 +99,7 @@ __device__ __forceinline__ float cell_cfl(const Box &b, const FieldPtrs &W, long
                    fmaxf(bzl * bzl, bzh * bzh);
   const float c = sqrtf((b.gamma * W.f[4][a] + b2) / W.f[0][a]);
   float t = 3.0e38f;
-  for (int d = 0; d < 3; ++d) t = fminf(t, b.cfl * b.d[d] / fabsf(W.f[1 + d][a] + c));
+  for (int d = 0; d < 3; ++d) t = fminf(t, b.cfl * b.d[d] / cfl_speed(b, W.f[1 + d][a], c));
   return t;
 }
 

@@ -274,7 +274,7 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
     recon_bsum(mode, BX[s0] + BX[s0 + 1], BX[s1] + BX[s1 + 1], BX[s2] + BX[s2 + 1],
                BX[s3] + BX[s3 + 1], L[6], Rr[6]);
     if (src) {
-      const float mbv = 0.5f * h * accel_y(b, 0.5f * (VX[s1] + VX[s2]), yrow);
+      const float mbv = 0.5f * h * accel_y_face(b, 0.5f * (VX[s1] + VX[s2]), yrow);
       L[1] += mbv;
       Rr[1] += mbv;
     }
@@ -316,7 +316,7 @@ __device__ __forceinline__ void faces_tile(const Box &b, const FieldPtrs &W, con
     if (src) {
       const float *VA = isy ? VX : VY;
       const float va = 0.5f * (VA[s1] + VA[s2]);
-      const float mbv = 0.5f * h * (isy ? accel_y(b, va, yrow) : accel_x(b, va));
+      const float mbv = 0.5f * h * (isy ? accel_y_face(b, va, yrow) : accel_x(b, va));
       L[1] += mbv;
       Rr[1] += mbv;
     }
@@ -783,9 +783,9 @@ __device__ __forceinline__ void update_tile(const Box &b, const FieldPtrs &Win, 
             const float b2 = fmaxf(bxl * bxl, bxh * bxh) + fmaxf(byl * byl, byh * byh) +
                              fmaxf(bzl * bzl, bzh * bzh);
             const float cs = sqrtf((b.gamma * pres + b2) / rho);
-            const float t = fminf(b.cfl * b.d[0] / fabsf(ux + cs),
-                                  fminf(b.cfl * b.d[1] / fabsf(uy + cs),
-                                        b.cfl * b.d[2] / fabsf(uz + cs)));
+            const float t = fminf(b.cfl * b.d[0] / cfl_speed(b, ux, cs),
+                                  fminf(b.cfl * b.d[1] / cfl_speed(b, uy, cs),
+                                        b.cfl * b.d[2] / cfl_speed(b, uz, cs)));
             cfl_t = fminf(cfl_t, t);
           }
         }

@@ -94,13 +94,14 @@ __global__ void __launch_bounds__(256) k_flux(Box b, FieldPtrs W, FluxOut F, con
   }
   if (b.source == 1 && D != 2) {
     // face predictor v_n += 0.5 h a_n, a from the two cells' mean velocity at
-    // the centre of the face's own index (INL:793-797)
+    // the centre of the face's own index (INL:793-797), or at the face
+    // itself with NMHD_FIX_FACE_POSITION
     const float h = stage_h(st, hsel);
     float acc;
     if (D == 0)
       acc = accel_x(b, 0.5f * (W.f[2][s[1]] + W.f[2][s[2]]));
     else
-      acc = accel_y(b, 0.5f * (W.f[1][s[1]] + W.f[1][s[2]]), c.v[1]);
+      acc = accel_y_face(b, 0.5f * (W.f[1][s[1]] + W.f[1][s[2]]), c.v[1]);
     const float mbv = 0.5f * h * acc;
     L[1] += mbv;
     R[1] += mbv;

@@ -296,19 +296,28 @@ void or_interpolate_batch(size_t n, int recon, const float *x0, const float *x1,
 
 /* RCo::position with the corner (-ext/2): every staggered position gets the
  * +0.5 offset because `Bit & 2 == 0` parses as `Bit & (2 == 0)`
- * (crystal.h:86-104).  Only the radial (Y) component is used. */
-static inline float local_position_y(const or_grid *g, int y_wrapped) {
+ * (crystal.h:86-104).  Only the radial (Y) component is used.  off is the
+ * offset in cells: 0.5 everywhere in the reference; 0 for a radial face's
+ * own position (fix bit 4). */
+static inline float local_position_y_at(const or_grid *g, int y_wrapped, float off) {
   const float corner = -g->ext[1] / 2;
-  return corner + g->d[1] * ((float)y_wrapped + 0.5f);
+  return corner + g->d[1] * ((float)y_wrapped + off);
+}
+
+/* KEP/mhd_solver.curb:64-81: the acceleration at radial position offset off
+ * of row y (0.5: the cell centre, and the reference's face position) */
+static inline void accel_at(const or_grid *g, float vX, float vY, int y, float off,
+                            float *a) {
+  const float Om = g->omega, q = g->qshear;
+  a[0] = -2.0f * Om * vY;
+  a[1] = 2.0f * Om * vX + 2.0f * q * sqf(Om) * local_position_y_at(g, y, off);
+  a[2] = 0;
 }
 
 /* KEP/mhd_solver.curb:64-81, cell-centred overload (Bit 7) */
 static inline void accel_cell(const or_grid *g, float vX, float vY, int y,
                               float *a) {
-  const float Om = g->omega, q = g->qshear;
-  a[0] = -2.0f * Om * vY;
-  a[1] = 2.0f * Om * vX + 2.0f * q * sqf(Om) * local_position_y(g, y);
-  a[2] = 0;
+  accel_at(g, vX, vY, y, 0.5f, a);
 }
 
 /* ------------------------------------------------------------ kernels */
@@ -328,8 +337,9 @@ void or_cfl_field(or_ctx *c, float goal_time, float *out) {
                      maxf_(sqf(w[OR_BZ][a]), sqf(w[OR_BZ][addr_off(g, idx, 0, 0, 1)]));
     for (int d = 0; d < 3; ++d) {
       const float len = g->d[d];
-      const float speed =
-          fabsf(w[OR_VX + d][a] + sqrtf((g->gamma * w[OR_P][a] + b2) / w[OR_RHO][a]));
+      /* |v + c| (INL:715-720); |v| + c with fix bit 1 */
+      const float v = (g->fixes & 1) ? fabsf(w[OR_VX + d][a]) : w[OR_VX + d][a];
+      const float speed = fabsf(v + sqrtf((g->gamma * w[OR_P][a] + b2) / w[OR_RHO][a]));
       min_time = minf_(min_time, g->cfl_factor * len / speed);
     }
     out[a] = min_time;
@@ -394,11 +404,12 @@ static void calc_flux(or_ctx *c, int d, float h, float *const *W) {
     if (g->source == 1 && d != 2) {
       /* INL:793-797 + the face overload of KEP external_acceleration:
        * v averaged over the two cells of the face, Y position at the centre
-       * of the face's own cell index (the +0.5 quirk). */
+       * of the face's own cell index (the +0.5 quirk), or at the face
+       * itself with fix bit 4. */
       const float vX = avgf(W[OR_VX][cc[1]], W[OR_VX][cc[2]]);
       const float vY = avgf(W[OR_VY][cc[1]], W[OR_VY][cc[2]]);
       float acc[3];
-      accel_cell(g, vX, vY, idx[1], acc);
+      accel_at(g, vX, vY, idx[1], (g->fixes & 4) ? 0.0f : 0.5f, acc);
       const float mbv = 0.5f * h * acc[d];
       L[1] += mbv;
       R[1] += mbv;
@@ -622,21 +633,28 @@ static void fill_y(or_ctx *c) {
 #pragma omp parallel for schedule(static)
   for (a = 0; a < unit; ++a) {
     const int sx = (int)(a % nx), sy = (int)((a / nx) % M), sz = (int)(a / nx / M);
-    int inc_x0 = sx, dec_x0 = sx;
+    int inc_x0 = sx, dec_x0 = sx, inc_x1 = sx, dec_x1 = sx;
     if (g->shear_y) {
       /* the reference's `% gSizeX` assumes 0 <= n < nx; wrap fully here */
       inc_x0 = ((sx + n) % nx + nx) % nx;
       dec_x0 = ((sx - n - 1) % nx + nx) % nx;
+      inc_x1 = ((sx + n + 1) % nx + nx) % nx; /* INL:101, 103 */
+      dec_x1 = ((sx - n) % nx + nx) % nx;
     }
-    const long src_inc = ((long)sz * ny + (ny - 2 * M + sy)) * nx + inc_x0;
-    const long src_dec = ((long)sz * ny + (M + sy)) * nx + dec_x0;
+    const long row_inc = ((long)sz * ny + (ny - 2 * M + sy)) * nx;
+    const long row_dec = ((long)sz * ny + (M + sy)) * nx;
     const long dst_inc = ((long)sz * ny + sy) * nx + sx;
     const long dst_dec = ((long)sz * ny + (ny - M + sy)) * nx + sx;
+    const int interp = g->shear_y && (g->fixes & 2);
     for (int f = 0; f < 8; ++f) {
-      float vi = c->w[f][src_inc], vd = c->w[f][src_dec];
+      float vi = c->w[f][row_inc + inc_x0], vd = c->w[f][row_dec + dec_x0];
       if (g->shear_y) {
-        float bi = (1.0f - dr) * vi + dr * vi;
-        float bd = dr * vd + (1.0f - dr) * vd;
+        /* the reference reads val_inc1 / val_dec1 (INL:104-107) but weights
+         * val_inc0 / val_dec0 twice (INL:108-110); fix bit 2 uses them */
+        const float vi1 = interp ? c->w[f][row_inc + inc_x1] : vi;
+        const float vd1 = interp ? c->w[f][row_dec + dec_x1] : vd;
+        float bi = (1.0f - dr) * vi + dr * vi1;
+        float bd = dr * vd + (1.0f - dr) * vd1;
         if (f == OR_VX) {
           bi = bi - relvel;
           bd = bd + relvel;

@@ -49,6 +49,9 @@ typedef struct or_grid {
   int shear_y;         /* Y ghost fill: 0 periodic copy, 1 shearing-periodic   */
   int recon;           /* 0: minbee2 (minmod MUSCL), 1: piecewiseconst1        */
   int integrator;      /* 0: midpoint2, 1: euler1                             */
+  int fixes;           /* corrections of reference quirks, a bit set (0: none):
+                        * 1 CFL speed |v| + c, 2 interpolating shear fill,
+                        * 4 face predictor at the face position             */
 } or_grid;
 
 typedef struct or_ctx or_ctx;

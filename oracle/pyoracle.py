@@ -30,6 +30,7 @@ class OrGrid(ctypes.Structure):
         ("shear_y", ctypes.c_int),
         ("recon", ctypes.c_int),
         ("integrator", ctypes.c_int),
+        ("fixes", ctypes.c_int),
     ]
 
 
@@ -138,6 +139,7 @@ def make_grid(cfg):
     g.shear_y = cfg["shear_y"]
     g.recon = cfg.get("recon", 0)
     g.integrator = cfg.get("integrator", 0)
+    g.fixes = cfg.get("fixes", 0)
     return g
 
 

@@ -76,6 +76,7 @@ def _bad_desc_cases():
          b"2^31")
     case("2^30-cell plane", lambda d: [d.global_size.__setitem__(i, v)
                                        for i, v in enumerate((32768, 32768, 1))], b"2^30")
+    case("fixes", lambda d: setattr(d, "fixes", 8), b"fixes")
     return base, cases
 
 

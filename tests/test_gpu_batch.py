@@ -80,6 +80,30 @@ def test_batch_eager_equals_single_boxes(gpu_ready):
     assert len(dts) > 1
 
 
+def test_batch_with_cfl_speed_fix_equals_single_boxes(gpu_ready):
+    """Boxes with NMHD_FIX_CFL_SPEED (|v| + c) in a batch, mixed with boxes
+    without it: each equals its single-box tiled context bit for bit, and the
+    fixed boxes take their own, shorter steps (each box record carries its
+    own corrections; a bulk flow of -0.3 along x makes |v| + c exceed
+    |v + c| in the fastest cells)."""
+    announce("batch with per-box CFL-speed fix")
+    boxes = []
+    for i, (cfg, W) in enumerate(small_boxes(seed=23)):
+        if i % 2:
+            W = W.copy()
+            W[1] -= np.float32(0.3)
+            cfg = dict(cfg, fixes=capi.FIX_CFL_SPEED)
+        boxes.append((cfg, W))
+    ref = single_runs(boxes, 1e30, 4)
+    B = make_batch(boxes)
+    k, _ = B.run(1e30, 4)
+    assert k == 4
+    assert_batch_equals(B, ref)
+    plain = single_runs([(dict(c, fixes=0), W) for c, W in boxes], 1e30, 4)
+    for i in range(1, len(boxes), 2):
+        assert ref[i][1][0] < plain[i][1][0], i
+
+
 def test_batch_graph_equals_eager(gpu_ready):
     announce("batch graph vs eager, 7 + 4 steps (every rotation phase, replays)")
     boxes = small_boxes(seed=3)

@@ -148,6 +148,75 @@ def test_strict_shearing_box_20_steps_bitwise(gpu_ready):
     assert g.get_time()[0] == o.time
 
 
+# ---------------------------- corrections of the quirks (nmhd_desc.fixes)
+
+def fixed_case(fixes):
+    cfg, W = case("sb16")
+    return dict(cfg, fixes=fixes), W
+
+
+@pytest.mark.parametrize("path", [capi.PATH_STRICT, capi.PATH_TILED])
+def test_fixes_ghosts_and_cfl_bitwise(gpu_ready, path):
+    """Interpolating shear fill and |v| + c CFL speed, on the device as on
+    the oracle, bit for bit (the ghost and CFL kernels are shared by every
+    path); each correction changes the result."""
+    announce("fixes ghost/cfl path %d" % path)
+    cfg, W = fixed_case(capi.FIX_ALL)
+    o, g = both(cfg, W, path=path)
+    q = po.Oracle(dict(cfg, fixes=0), W)
+    # the same ghost fill without the CFL correction
+    r = po.Oracle(dict(cfg, fixes=capi.FIX_ALL & ~capi.FIX_CFL_SPEED), W)
+    for t in (0.0, 0.1234, 0.63, 5.5):
+        for x in (o, g, q, r):
+            x.set_time(t)
+            x.fill_ghosts()
+        assert bitwise_equal(g.get_state(), o.get_state()), t
+        assert bitwise_equal(np.float32(g.cfl_dt(10.0)), np.float32(o.cfl_dt(10.0)))
+        assert not bitwise_equal(q.get_state(), o.get_state()), t
+        # on the same cells |v| + c >= |v + c|: never a longer step (strictly
+        # shorter where the fastest cell moves against the sound wave; CPU
+        # test of the oracle)
+        assert bitwise_equal(r.get_state(), o.get_state())
+        assert r.cfl_dt(10.0) >= o.cfl_dt(10.0)
+
+
+def test_fixes_strict_steps_bitwise(gpu_ready):
+    """All three corrections on the strict path: right-hand side and five
+    steps of the shearing box equal the oracle bit for bit."""
+    announce("fixes strict steps")
+    cfg, W = fixed_case(capi.FIX_ALL)
+    o, g = both(cfg, W, path=capi.PATH_STRICT)
+    assert bitwise_equal(g.rhs(0.003), o.rhs(0.003))
+    for k in range(5):
+        dto, _ = o.step()
+        dtg, _ = g.step()
+        assert dto == dtg
+        assert bitwise_equal(g.get_state(), o.get_state()), k
+    q = po.Oracle(dict(cfg, fixes=0), W)
+    q.run(1e30, 5)
+    assert not bitwise_equal(q.get_state(), o.get_state())
+
+
+@pytest.mark.parametrize("fixes", [capi.FIX_FACE_POSITION, capi.FIX_ALL])
+def test_fixes_fast_steps(gpu_ready, fixes):
+    """The tiled path with corrections against the oracle with the same
+    corrections, to the fast path's per-step tolerance."""
+    announce("fixes %d tiled steps" % fixes)
+    cfg, W = fixed_case(fixes)
+    o, g = both(cfg, W, path=capi.PATH_TILED)
+    for k in range(5):
+        dto, _ = o.step()
+        dtg, _ = g.step()
+        assert abs(dtg - dto) <= 1e-5 * dto
+        a, b = g.get_state(), o.get_state()
+        err = group_rel_err(a, b)
+        frac = group_frac_within(a, b, 1e-5 * (k + 1))
+        print("step %d group rel err %s, frac within %.0e: %.6f"
+              % (k, np.array2string(err, precision=2), 1e-5 * (k + 1), frac))
+        assert frac >= 0.999
+        assert err.max() < 1e-3
+
+
 # ------------------------------------------------ fast path: tolerance
 
 FAST_PATHS = [capi.PATH_TILED, capi.PATH_UNFUSED]

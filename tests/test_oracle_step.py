@@ -379,3 +379,92 @@ def test_tidal_predictor_sits_at_cell_centre_above_radial_face():
     assert np.abs(F - want_face[None, :, None]).min() > 0.02 * scale
     # azimuthal faces: a_az = -2 Omega v_rad = 0 at rest, no mass flux
     assert np.abs(o.face_flux(0, h)[0]).max() <= 1e-12
+
+
+# ------------------- corrections of the quirks (nmhd_desc.fixes), on the oracle
+
+def test_fix_cfl_speed_uses_speed_plus_sound_speed():
+    """Fix bit 1: the CFL speed is |v_d| + c instead of |v_d + c|
+    (mhd_solver.inl:715-720).  Uniform rho = p = 1, B = 0 (c = sqrt(gamma))
+    with v_x = -u: the reference takes |c - u| along x, so the minimum comes
+    from y or z at C d / c; the correction takes u + c along x, C d / (u + c).
+    With v_x = +u both agree."""
+    f32 = np.float32
+    cfg = P.config_c1(8)
+    u = f32(0.5)
+    c = np.sqrt(f32(cfg["gamma"]) * f32(1.0) / f32(1.0))
+    d = f32(cfg["d"][0])
+    C = f32(cfg["cfl"])
+    for sign in (-1, 1):
+        W = P.empty_state(cfg)
+        W[0] = 1.0
+        W[4] = 1.0
+        W[1] = sign * u
+        quirk = po.Oracle(cfg, W).cfl_dt(1e30)
+        fixed = po.Oracle(dict(cfg, fixes=1), W).cfl_dt(1e30)
+        if sign < 0:
+            assert f32(quirk) == C * d / c, quirk
+            assert f32(fixed) == C * d / (u + c), fixed
+            assert fixed < quirk
+        else:
+            assert f32(quirk) == f32(fixed) == C * d / np.abs(u + c)
+
+
+def test_fix_shear_interp_moves_rows_by_the_fractional_displacement():
+    """Fix bit 2: the shearing-periodic fill interpolates between the two
+    source cells with weights 1-d, d (the gather kernel's val_inc1 /
+    val_dec1, mhd_solver.inl:100-110), so a field linear in the azimuthal
+    index arrives shifted by exactly the displacement D/dx = n + d: the lower
+    ghost rows hold the top interior rows at x + D/dx, the upper ghost rows
+    the bottom interior rows at x - D/dx.  The reference copies the cell at
+    the floor, x + n and x - n - 1 (one cell off on the upper side)."""
+    cfg = sb_config(16, 16, 8)
+    nx, ny, M = cfg["n"][0], cfg["n"][1], cfg["m"][1]
+    t = 0.63
+    _, n, dr = po.relative_displacement(cfg, t)
+    assert n == 0 and 0.1 < dr < 0.9, (n, dr)
+    W = P.empty_state(cfg)
+    x = np.arange(nx, dtype=np.float32)
+    W[0] = (1.0 + 0.01 * x)[None, None, :].astype(np.float32)
+    W[4] = 1.0
+    shift = n + dr
+    for fixes, lo_want, hi_want in (
+            (2, 1.0 + 0.01 * (x + shift), 1.0 + 0.01 * (x - shift)),
+            (0, 1.0 + 0.01 * (x + n), 1.0 + 0.01 * (x - n - 1))):
+        o = po.Oracle(dict(cfg, fixes=fixes), W)
+        o.set_time(t)
+        o.fill_ghosts()
+        rho = o.get_state()[0]
+        # cells whose source cells do not wrap around the azimuthal period
+        lo = slice(0, nx - n - 1)
+        hi = slice(n + 1, nx)
+        for j in range(M):
+            np.testing.assert_allclose(rho[:, j, lo], np.broadcast_to(lo_want[lo], rho[:, j, lo].shape),
+                                       rtol=2e-7, atol=0)
+            np.testing.assert_allclose(rho[:, ny - M + j, hi],
+                                       np.broadcast_to(hi_want[hi], rho[:, ny - M + j, hi].shape),
+                                       rtol=2e-7, atol=0)
+
+
+def test_fix_face_position_puts_the_tidal_term_on_the_face():
+    """Fix bit 4: the face predictor evaluates 2 q Omega^2 Y at the radial
+    face's own position, -L/2 + dY j, instead of the centre of its cell half
+    a cell further out (crystal.h:86-104).  Same state as the quirk's test:
+    v = 0, B = 0, uniform rho, p; the radial-face mass flux of row j is
+    rho h/2 2 q Omega^2 Y_face(j)."""
+    cfg = dict(sb_config(16, 16, 4), fixes=4)
+    rho = 1.3
+    W = P.empty_state(cfg)
+    W[0] = rho
+    W[4] = 0.9
+    o = po.Oracle(cfg, W)
+    h = 0.01
+    F = o.face_flux(1, h)[0].astype(np.float64)
+    ny, dy, ext = cfg["n"][1], cfg["d"][1], cfg["ext"][1]
+    q, om = cfg["qshear"], cfg["omega"]
+    y_face = -ext / 2 + dy * np.arange(ny)
+    want = rho * 0.5 * h * 2.0 * q * om * om * y_face
+    scale = np.abs(want).max()
+    assert np.abs(F - want[None, :, None]).max() <= 1e-5 * scale
+    want_centre = rho * 0.5 * h * 2.0 * q * om * om * (y_face + dy / 2)
+    assert np.abs(F - want_centre[None, :, None]).min() > 0.02 * scale
