@@ -5,7 +5,10 @@
 // which case it saves the "c" checkpoint and prints "ToBeContinued".
 //
 //   kepler_disk [--new|--continue] [-o DIR] [-n N] [-g GOAL] [-w SECONDS]
-//               [--beta B] [--steps K]
+//               [--beta B] [--steps K] [--fixes BITS]
+//
+// --fixes takes enum nmhd_fix bits (corrections of reference quirks; the
+// default 0 runs the reference's behaviour).
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -19,7 +22,7 @@ using namespace nmhd;
 int main(int argc, char **argv) {
   bool fresh = true;
   std::string outdir = "stat/kepler";
-  int n = 64, max_steps = -1;
+  int n = 64, max_steps = -1, fixes = 0;
   double goal = 1.0, wall = -1, beta = 400.0;
   for (int i = 1; i < argc; ++i) {
     const std::string a = argv[i];
@@ -32,6 +35,7 @@ int main(int argc, char **argv) {
     else if (a == "-w") wall = std::atof(next().c_str());
     else if (a == "--beta") beta = std::atof(next().c_str());
     else if (a == "--steps") max_steps = std::atoi(next().c_str());
+    else if (a == "--fixes") fixes = std::atoi(next().c_str());
     else {
       std::fprintf(stderr, "unknown option %s\n", a.c_str());
       return 2;
@@ -43,6 +47,7 @@ int main(int argc, char **argv) {
   cfg.cfl_factor = 0.4f;
   cfg.source = NMHD_SOURCE_SHEARING_BOX;
   cfg.shearing_boundary = true;
+  cfg.fixes = fixes;
   LocalSetup local(0, outdir);
   MHDSolver solver(cfg, local);
   const Real b0 = Real(std::sqrt(2.0 / beta));

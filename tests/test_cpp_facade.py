@@ -77,3 +77,20 @@ def test_kepler_disk_app_continue(gpu_ready, tmp_path):
     sx, sy, sz, fields = PP.read_bin(os.path.join(str(tmp_path), "c", "0000.bin"))
     assert grid.size == [16, 16, 16] and (sx, sy, sz) == (16, 24, 16)
     assert np.allclose(arrays, fields[:, :, 4:20, :], rtol=0, atol=0)
+
+
+@pytest.mark.gpu
+def test_kepler_disk_app_with_fixes(gpu_ready, tmp_path):
+    """SimulationConfig::fixes reaches the device: the driver with all three
+    corrections (--fixes 7) runs, and its state after 3 steps differs from
+    the reference-behaviour run of the same setup."""
+    print("\n[case] kepler_disk app --fixes", flush=True)
+    app = os.path.join(ROOT, "nmhd_amd", "bin", "kepler_disk")
+    out = {}
+    for fx in ("0", "7"):
+        d = tmp_path / ("f" + fx)
+        r = run(app, "--new", "-o", str(d), "-n", "16", "--steps", "3", "--fixes", fx)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "generation = 3" in r.stdout
+        out[fx] = open(os.path.join(str(d), "c", "0000.bin"), "rb").read()
+    assert len(out["0"]) == len(out["7"]) and out["0"] != out["7"]
