@@ -87,8 +87,9 @@ __device__ __forceinline__ void step_commit(StepState &s) {
 }
 
 // CFL time of one cell (cfl_condition_inner, INL:714-720):
-// min_d C*d_d/|v_d + sqrt((gamma p + b2)/rho)|, b2 = sum_d max(B_lo^2, B_hi^2),
-// without FMA contraction so that every CFL pass rounds the same way.
+// min_d C*d_d/|v_d + sqrt((gamma p + b2)/rho)|, b2 = sum_d max(B_lo^2, B_hi^2)
+// (|v_d| + sqrt(..) with fix_cfl), without FMA contraction so that every CFL
+// pass rounds the same way.
 __device__ __forceinline__ float cell_cfl(const Box &b, const FieldPtrs &W, long a, int x, int y,
                                           int z) {
 #pragma clang fp contract(off)
