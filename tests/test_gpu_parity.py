@@ -137,9 +137,12 @@ def test_strict_c1_100_steps_bitwise(gpu_ready):
     assert g.get_time()[0] == o.time
 
 
-def test_strict_shearing_box_20_steps_bitwise(gpu_ready):
-    announce("strict shearing box 32^3, 20 steps")
-    cfg = P.config_c2((32, 32, 32))
+@pytest.mark.parametrize("fixes", [0, capi.FIX_ALL])
+def test_strict_shearing_box_20_steps_bitwise(gpu_ready, fixes):
+    """20 steps of the 32^3 shearing box, as the reference computes them and
+    with all three quirk corrections."""
+    announce("strict shearing box 32^3, 20 steps, fixes %d" % fixes)
+    cfg = dict(P.config_c2((32, 32, 32)), fixes=fixes)
     W = P.shearing_box_mri(cfg, seed=1002)
     o, g = both(cfg, W, path=capi.PATH_STRICT)
     o.run(1e30, 20)
