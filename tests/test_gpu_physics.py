@@ -39,9 +39,12 @@ def announce(msg):
     print("\n[case] " + msg, flush=True)
 
 
-def test_mri_growth_rate(gpu_ready):
-    announce("shearing box 64^3, beta 400: MRI growth to t = 15")
-    cfg = P.config_c2((64, 64, 64))
+@pytest.mark.parametrize("fixes", [0, capi.FIX_ALL])
+def test_mri_growth_rate(gpu_ready, fixes):
+    """As the reference computes the step, and with the three quirk
+    corrections (nmhd_desc.fixes), which must not change the physics."""
+    announce("shearing box 64^3, beta 400: MRI growth to t = 15, fixes %d" % fixes)
+    cfg = dict(P.config_c2((64, 64, 64)), fixes=fixes)
     g = capi.Context(cfg)
     g.set_state(P.shearing_box_mri(cfg, seed=1002))
     rec = {}
