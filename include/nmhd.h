@@ -237,16 +237,17 @@ int nmhd_set_tiles(nmhd_ctx *ctx, int faces_tile, int update_tile);
 int nmhd_get_tiles(const nmhd_ctx *ctx, int *faces_tile, int *update_tile);
 
 /* ------------------------------------------------------------------------
- * Batches: many independent periodic boxes of uneven sizes advanced
- * together.  No reference counterpart: the reference runs one
+ * Batches: many independent boxes of uneven sizes advanced together.  No reference counterpart: the reference runs one
  * compile-time-sized box per process (mhd_solver.h:514-640), so its
  * equivalent is one MHDSolver::proceed (INL:1086-1204) per box, run one
  * after another.  Every box keeps its own time, CFL dt and goal clipping;
  * its results are bit-identical to the same box advanced alone by an
  * nmhd_ctx on NMHD_PATH_TILED.  Each RK stage is two persistent launches
  * over a work table of tiles from all boxes; runs replay the step as a HIP
- * graph.  Boxes must be periodic: margin {0,0,0}, source NONE, shear_y 0,
- * and all boxes share one integrator.
+ * graph.  Any descriptor a context accepts may be batched (periodic and
+ * shearing boxes, ghost layers, sources and quirk corrections per box), at
+ * most 32767 cells per axis including ghosts; all boxes share one
+ * integrator.
  * ---------------------------------------------------------------------- */
 typedef struct nmhd_batch nmhd_batch;
 

@@ -1,5 +1,6 @@
 // capi_batch.hip -- the nmhd_batch_* entry points of include/nmhd.h: many
-// independent periodic boxes in shared pools, per-box step records, a work
+// independent boxes (periodic or shearing) in shared pools, per-box step
+// records, per-box ghost fills, a work
 // table of tiles for the stage kernels (grouped by tile shape: one launch per
 // shape with a workgroup per tile, or one persistent launch over the whole
 // table), HIP-graph replay of the step, and the autotuner of tile shapes and
@@ -69,6 +70,9 @@ struct nmhd_batch {
   size_t cap_f = 0, cap_u = 0, cap_t = 0;
   bool items_dirty = true;
   bool cfl_valid = false;
+  // blocks per box of the ghost fills and their CFL pass (0: no box has
+  // that ghost layer)
+  int grid_gy = 0, grid_gz = 0, grid_gcfl = 0;
   hipGraphExec_t exec[3] = {};
   std::string err;
 };
@@ -222,7 +226,14 @@ int enqueue_step(nmhd_batch *b, bool full_cfl, float goal) {
   hipStream_t s = b->stream;
   const int A = b->cur, n = b->nbox;
   launch_batch_begin(s, b->st, n, full_cfl);
-  if (full_cfl) launch_batch_cfl(s, b->d_boxes, n, b->state[A], goal, b->st);
+  // ghost fills of the boxes that have ghost layers, then the CFL minimum:
+  // a full pass, or the ghost cells added to the stage-2 epilogue's minimum
+  if (b->grid_gy) launch_batch_ghost_y(s, b->d_boxes, n, b->grid_gy, b->state[A], b->st);
+  if (b->grid_gz) launch_batch_ghost_z(s, b->d_boxes, n, b->grid_gz, b->state[A], b->st);
+  if (full_cfl)
+    launch_batch_cfl(s, b->d_boxes, n, b->state[A], goal, b->st);
+  else if (b->grid_gcfl)
+    launch_batch_cfl_ghosts(s, b->d_boxes, n, b->grid_gcfl, b->state[A], b->st);
   launch_batch_dt(s, b->st, n, b->d_goal);
   auto stage = [&](int in, int base, int out, int hsel, bool last) {
     const int r = faces_over(b, s, b->fwg, b->d_fitems, b->nf, b->nfv, b->state[in], b->st, hsel);
@@ -539,15 +550,12 @@ int nmhd_batch_create(const nmhd_desc *descs, int nbox, nmhd_batch **out) {
     std::string why;
     const nmhd_desc &d = descs[i];
     if (!desc_valid(&d, why)) return bfail(nullptr, NMHD_EINVAL, "box " + std::to_string(i) + ": " + why);
-    if (d.margin[0] || d.margin[1] || d.margin[2] || d.source != NMHD_SOURCE_NONE || d.shear_y)
-      return bfail(nullptr, NMHD_EINVAL,
-                   "box " + std::to_string(i) +
-                       ": batched boxes are periodic (margin 0, source NONE, shear_y 0)");
     if (d.integrator != descs[0].integrator)
       return bfail(nullptr, NMHD_EINVAL, "all boxes of a batch share one integrator");
     for (int a = 0; a < 3; ++a)
-      if (d.global_size[a] > 32767)
-        return bfail(nullptr, NMHD_EINVAL, "batched boxes are at most 32767 cells per axis");
+      if (d.global_size[a] + 2 * d.margin[a] > 32767)
+        return bfail(nullptr, NMHD_EINVAL,
+                     "batched boxes are at most 32767 cells per axis (ghosts included)");
   }
   nmhd_batch *b = new nmhd_batch();
   auto bail = [&](int code, const std::string &msg) {
@@ -574,6 +582,26 @@ int nmhd_batch_create(const nmhd_desc *descs, int nbox, nmhd_batch **out) {
   }
   b->total8 = off8;
   b->total21 = off21;
+  // ghost grids: blocks of 256 for the largest slab, at most 512 per box
+  {
+    long gy = 0, gz = 0, gc = 0;
+    for (const BatchBox &bb : b->boxes) {
+      const Box &x = bb.b;
+      gy = std::max(gy, (long)x.n[0] * x.m[1] * x.n[2]);
+      gz = std::max(gz, (long)x.n[0] * x.n[1] * x.m[2]);
+      long c = 0;
+      if (x.m[1]) c += (long)x.n[0] * 2 * x.m[1] * x.n[2];
+      if (x.m[2]) c += (long)x.n[0] * x.n[1] * (2 * x.m[2] + 1);
+      gc = std::max(gc, c);
+    }
+    auto blocks = [](long cells, long cap) {
+      return cells == 0 ? 0 : (int)std::min(cap, (cells + 255) / 256);
+    };
+    b->grid_gy = blocks(gy, 512);
+    b->grid_gz = blocks(gz, 512);
+    // each block of the CFL pass ends in one atomic on its box's record
+    b->grid_gcfl = blocks(gc / 4, 128);
+  }
   if (hipGetDevice(&b->device) != hipSuccess) return bail(NMHD_EHIP, "no HIP device");
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, b->device) != hipSuccess)

@@ -1,4 +1,4 @@
-// kernels_batch.hip -- many independent periodic boxes of uneven sizes
+// kernels_batch.hip -- many independent boxes of uneven sizes
 // advanced together.  Each RK stage runs the face fluxes, then the update,
 // over a work table of tiles from every box, either as one workgroup per
 // work item with one launch per tile shape (the kernel then needs only that
@@ -8,11 +8,14 @@
 // fields in the batch's pools.  The tile functions are the
 // single-box path's (tiled_tiles.h), so a box advanced in a batch is
 // bit-identical to the same box advanced alone.  The bookkeeping kernels run
-// one thread per box on the per-box step records.
+// one thread per box on the per-box step records; boxes with ghost layers
+// (the shearing box) are filled, and their ghost cells' CFL reduced, by
+// per-box grids over the single-box cell functions (ghost_fill.h).
 //
 // The reference advances one compile-time-sized box per process and has no
 // counterpart of this path; its per-box arithmetic is that of proceed()
 // (library/mhd_solver.inl:1086-1204).
+#include "ghost_fill.h"
 #include "tiled_tiles.h"
 
 namespace nmhd {
@@ -182,6 +185,58 @@ __global__ void __launch_bounds__(256) k_batch_cfl(const BatchBox *boxes, float 
   }
 }
 
+// Ghost fills of the boxes that have ghost layers (shearing-periodic or
+// periodic Y rows, periodic Z planes): blockIdx.y is the box, blocks stride
+// over its slab; the cell operations are the single-box kernels'.
+__global__ void k_batch_ghost_y(const BatchBox *boxes, float *state, const StepState *st) {
+  const int box = blockIdx.y;
+  const BatchBox bb = boxes[box];
+  if (bb.b.m[1] == 0 || st[box].finished) return;
+  const FieldPtrs W = box_fields(state, bb);
+  const long unit = (long)bb.b.n[0] * bb.b.m[1] * bb.b.n[2];
+  const double t = st[box].t;
+  for (long a = (long)blockIdx.x * blockDim.x + threadIdx.x; a < unit;
+       a += (long)gridDim.x * blockDim.x)
+    ghost_y_cell(bb.b, W, t, a);
+}
+
+__global__ void k_batch_ghost_z(const BatchBox *boxes, float *state, const StepState *st) {
+  const int box = blockIdx.y;
+  const BatchBox bb = boxes[box];
+  if (bb.b.m[2] == 0 || st[box].finished) return;
+  const FieldPtrs W = box_fields(state, bb);
+  const long unit = (long)bb.b.n[0] * bb.b.n[1] * bb.b.m[2];
+  for (long a = (long)blockIdx.x * blockDim.x + threadIdx.x; a < unit;
+       a += (long)gridDim.x * blockDim.x)
+    ghost_z_cell(bb.b, W, a);
+}
+
+// CFL minimum over the cells the ghost fills rewrote, added to each box's
+// running minimum (the stage-2 epilogue reduced the others).
+__global__ void __launch_bounds__(256) k_batch_cfl_ghosts(const BatchBox *boxes, float *state,
+                                                          StepState *st) {
+  const int box = blockIdx.y;
+  const BatchBox bb = boxes[box];
+  const long n0 = ghost_cfl_count(bb.b, 0), n1 = ghost_cfl_count(bb.b, 1);
+  if ((n0 == 0 && n1 == 0) || st[box].finished) return;
+  __shared__ float red[4];
+  const FieldPtrs W = box_fields(state, bb);
+  float t = 3.0e38f;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n0;
+       i += (long)gridDim.x * blockDim.x)
+    t = fminf(t, ghost_cfl_cell(bb.b, W, 0, i));
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n1;
+       i += (long)gridDim.x * blockDim.x)
+    t = fminf(t, ghost_cfl_cell(bb.b, W, 1, i));
+  const float m = tiled::wave_min64(t);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float r = fminf(fminf(red[0], red[1]), fminf(red[2], red[3]));
+    atomicMin(&st[box].cfl_bits, __float_as_uint(r));
+  }
+}
+
 // Carry the state of boxes whose step is inactive (goal reached) into the
 // next rotation slot.
 __global__ void k_batch_copy(const BatchBox *boxes, float *src, float *dst, const StepState *st) {
@@ -209,6 +264,18 @@ void launch_batch_dt(hipStream_t s, StepState *st, int nbox, const float *goal) 
 }
 void launch_batch_commit(hipStream_t s, StepState *st, int nbox) {
   hipLaunchKernelGGL(k_batch_commit, dim3(grid_for(nbox, 64)), dim3(64), 0, s, st, nbox);
+}
+void launch_batch_ghost_y(hipStream_t s, const BatchBox *boxes, int nbox, int grid, float *state,
+                          const StepState *st) {
+  hipLaunchKernelGGL(k_batch_ghost_y, dim3(grid, nbox), dim3(256), 0, s, boxes, state, st);
+}
+void launch_batch_ghost_z(hipStream_t s, const BatchBox *boxes, int nbox, int grid, float *state,
+                          const StepState *st) {
+  hipLaunchKernelGGL(k_batch_ghost_z, dim3(grid, nbox), dim3(256), 0, s, boxes, state, st);
+}
+void launch_batch_cfl_ghosts(hipStream_t s, const BatchBox *boxes, int nbox, int grid,
+                             float *state, StepState *st) {
+  hipLaunchKernelGGL(k_batch_cfl_ghosts, dim3(grid, nbox), dim3(256), 0, s, boxes, state, st);
 }
 void launch_batch_copy(hipStream_t s, const BatchBox *boxes, int nbox, float *src, float *dst,
                        const StepState *st) {

@@ -10,7 +10,7 @@
 //               (INL:1111-1116), as one pass with a wave/block/grid min
 //   k_dt        dt clipping of proceed() (INL:1126-1137)
 //   k_commit    GlobalSetup::proceed (mhd_solver.hrb:56-59)
-#include "nmhd_kernels.h"
+#include "ghost_fill.h"
 
 namespace nmhd {
 
@@ -25,97 +25,26 @@ __global__ void k_open_step(StepState *st, int commit_prev, int full, int dt_too
   if (dt_too) step_dt(*st, goal);
 }
 
-// relative_displacement (INL:288-295) with D0 = 0 and
-// relvel = -q*Omega*kLocalExtentY (application/kepler_disk_demo/
-// mhd_solver.curb:46-52), in FP64 as the reference evaluates it.
-__host__ __device__ inline void displacement(const Box &b, double t, float *normalized,
-                                             int *n, float *minor) {
-  const float relvel = -b.qshear * b.omega * b.local_ext[1];
-  const double d0 = 0.0 + t * (double)relvel;
-  const int fold = (int)floor(d0 / (double)b.ext[0]);
-  const double d1 = d0 - (double)((float)fold * b.ext[0]);
-  *normalized = (float)d1;
-  *n = (int)floor(d1 / (double)b.d[0]);
-  *minor = (float)(d1 / (double)b.d[0] - *n);
-}
-
 void host_displacement(const Box &b, double t, float *normalized, int *n, float *minor) {
   displacement(b, t, normalized, n, minor);
 }
 
-// One thread per (x, row, z) of the M-row ghost slab: the lower ghost row j
-// takes the top interior row j shifted by +n along X, the upper ghost row
-// takes the bottom interior row shifted by -n-1, v_X offset by -/+ relvel;
-// the first upper ghost By face (the outer radial face) is kept.  With
-// fix_interp the shifted values are interpolated between the cells at +n
-// and +n+1 (-n-1 and -n) with the gather kernel's weights (INL:100-110).
+// Y ghost rows (ghost_y_cell), one thread per (x, row, z) of the slab.
 __global__ void k_ghost_y(Box b, FieldPtrs W, const StepState *st) {
-#pragma clang fp contract(off)
   if (st->finished) return;
-  const int nx = b.n[0], ny = b.n[1], M = b.m[1];
-  const long unit = (long)nx * M * b.n[2];
+  const long unit = (long)b.n[0] * b.m[1] * b.n[2];
   const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= unit) return;
-  // 32-bit index arithmetic: the slab holds fewer than 2^31 cells
-  const unsigned u = (unsigned)a, row = u / (unsigned)nx;
-  const int sx = (int)(u - row * (unsigned)nx);
-  const int sz = (int)(row / (unsigned)M);
-  const int sy = (int)(row - (unsigned)sz * (unsigned)M);
-  int n = 0;
-  float dr = 0.0f, nd, relvel = 0.0f;
-  int ix = sx, dx = sx, ix1 = sx, dx1 = sx;
-  if (b.shear_y) {
-    displacement(b, st->t, &nd, &n, &dr);
-    relvel = -b.qshear * b.omega * b.local_ext[1];
-    n %= nx;  // same shift, and no int overflow in sx + n for huge t
-    ix = ((sx + n) % nx + nx) % nx;
-    dx = ((sx - n - 1) % nx + nx) % nx;
-    ix1 = ix + 1 < nx ? ix + 1 : 0;
-    dx1 = dx + 1 < nx ? dx + 1 : 0;
-  }
-  const long src_inc = addr(b, ix, ny - 2 * M + sy, sz);
-  const long src_dec = addr(b, dx, M + sy, sz);
-  const long src_inc1 = addr(b, ix1, ny - 2 * M + sy, sz);
-  const long src_dec1 = addr(b, dx1, M + sy, sz);
-  const long dst_inc = addr(b, sx, sy, sz);
-  const long dst_dec = addr(b, sx, ny - M + sy, sz);
-  const bool interp = b.shear_y && b.fix_interp;
-  for (int f = 0; f < 8; ++f) {
-    float vi = W.f[f][src_inc];
-    float vd = W.f[f][src_dec];
-    // the second source cells (the reference's val_inc1 / val_dec1)
-    const float vi1 = interp ? W.f[f][src_inc1] : vi;
-    const float vd1 = interp ? W.f[f][src_dec1] : vd;
-    if (b.shear_y) {
-      // (1-d)*v + d*v, exactly as the gather kernel writes it (INL:108-110);
-      // with fix_interp (1-d)*v0 + d*v1 and d*v0 + (1-d)*v1
-      float bi = (1.0f - dr) * vi + dr * vi1;
-      float bd = dr * vd + (1.0f - dr) * vd1;
-      if (f == 1) {
-        bi = bi - relvel;
-        bd = bd + relvel;
-      }
-      vi = bi;
-      vd = bd;
-    }
-    W.f[f][dst_inc] = vi;
-    if (!(f == 6 && sy == 0)) W.f[f][dst_dec] = vd;
-  }
+  ghost_y_cell(b, W, st->t, a);
 }
 
-// Z ghost planes: [0, M) <- [nz-2M, nz-M), [nz-M, nz) <- [M, 2M).
+// Z ghost planes (ghost_z_cell).
 __global__ void k_ghost_z(Box b, FieldPtrs W, const StepState *st) {
   if (st->finished) return;
-  const int M = b.m[2];
-  const long plane = (long)b.n[0] * b.n[1];
-  const long unit = plane * M;
+  const long unit = (long)b.n[0] * b.n[1] * b.m[2];
   const long a = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (a >= unit) return;
-  for (int f = 0; f < 8; ++f) {
-    float *p = W.f[f];
-    p[a] = p[b.ncell - 2 * unit + a];
-    p[b.ncell - unit + a] = p[unit + a];
-  }
+  ghost_z_cell(b, W, a);
 }
 
 __device__ __forceinline__ float wave_min(float v) {
@@ -155,34 +84,14 @@ __global__ void __launch_bounds__(256) k_cfl(Box b, FieldPtrs W, float goal, Ste
   cfl_block_fold(t, st);
 }
 
-// CFL over the cells a ghost fill rewrites: the Y ghost rows, and with Z
-// ghosts the Z ghost planes plus the last interior plane (whose upper B_z
-// face is a ghost face).  slab 0: nx * 2My * nz cells, slab 1:
-// nx * ny * (2Mz + 1) cells.
+// CFL over the cells a ghost fill rewrites (ghost_cfl_cell), slab 0 or 1.
 __global__ void __launch_bounds__(256) k_cfl_ghosts(Box b, FieldPtrs W, int slab, StepState *st) {
   if (st->finished) return;
-  const int nx = b.n[0], ny = b.n[1], nz = b.n[2];
-  const int M = slab == 0 ? b.m[1] : b.m[2];
-  const long n = slab == 0 ? (long)nx * 2 * M * nz : (long)nx * ny * (2 * M + 1);
+  const long n = ghost_cfl_count(b, slab);
   float t = 3.0e38f;
   for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += (long)gridDim.x * blockDim.x) {
-    // 32-bit index arithmetic: a box holds fewer than 2^31 cells
-    const unsigned u = (unsigned)i, row = u / (unsigned)nx;
-    const int x = (int)(u - row * (unsigned)nx);
-    int y, z;
-    if (slab == 0) {
-      const unsigned m2 = 2u * (unsigned)M;
-      z = (int)(row / m2);
-      const int r = (int)(row - (unsigned)z * m2);
-      y = r < M ? r : ny - 2 * M + r;
-    } else {
-      const int r = (int)(row / (unsigned)ny);
-      y = (int)(row - (unsigned)r * (unsigned)ny);
-      z = r < M ? r : nz - 2 * M - 1 + r;
-    }
-    t = fminf(t, cell_cfl(b, W, addr(b, x, y, z), x, y, z));
-  }
+       i += (long)gridDim.x * blockDim.x)
+    t = fminf(t, ghost_cfl_cell(b, W, slab, i));
   cfl_block_fold(t, st);
 }
 

@@ -180,6 +180,14 @@ void launch_batch_dt(hipStream_t s, StepState *st, int nbox, const float *goal);
 void launch_batch_commit(hipStream_t s, StepState *st, int nbox);
 void launch_batch_copy(hipStream_t s, const BatchBox *boxes, int nbox, float *src, float *dst,
                        const StepState *st);
+// ghost fills and their CFL minimum for the boxes with ghost layers (grid
+// blocks per box, blockIdx.y the box; boxes without return at once)
+void launch_batch_ghost_y(hipStream_t s, const BatchBox *boxes, int nbox, int grid, float *state,
+                          const StepState *st);
+void launch_batch_ghost_z(hipStream_t s, const BatchBox *boxes, int nbox, int grid, float *state,
+                          const StepState *st);
+void launch_batch_cfl_ghosts(hipStream_t s, const BatchBox *boxes, int nbox, int grid,
+                             float *state, StepState *st);
 void launch_batch_faces(hipStream_t s, int grid, const BatchBox *boxes, const WorkItem *items,
                         int nitems, float *state, float *flux, const StepState *st, int hsel);
 void launch_batch_update(hipStream_t s, int grid, const BatchBox *boxes, const WorkItem *items,

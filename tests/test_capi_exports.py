@@ -41,16 +41,20 @@ def test_create_rejects_bad_descriptor_without_gpu():
     assert b"margin[0]" in capi.load().nmhd_last_error(None)
 
 
-def test_batch_create_rejects_non_periodic_without_gpu():
-    """Batched boxes must be periodic; checked before any device call."""
+def test_batch_create_validates_every_box_without_gpu():
+    """Each box's descriptor is checked as a context's is, and the batch's
+    own limits (one integrator, 32767 cells per axis with ghosts), before
+    any device call."""
     from nmhd_amd import problems as P
     good = P.make_config((16, 16, 16))
-    bad = P.config_c2((16, 16, 16))
     import pytest
-    with pytest.raises(capi.NmhdError, match="periodic"):
-        capi.Batch([good, bad])
     with pytest.raises(capi.NmhdError, match="integrator"):
         capi.Batch([good, P.make_config((8, 8, 8), integrator=1)])
+    with pytest.raises(capi.NmhdError, match="32767"):
+        capi.Batch([good, P.make_config((4, 32764, 4), margin=(0, 2, 0))])
+    bad = dict(P.config_c2((16, 16, 16)), fixes=8)
+    with pytest.raises(capi.NmhdError, match="box 1: unknown bit in fixes"):
+        capi.Batch([good, bad])
 
 
 def _bad_desc_cases():

@@ -1,6 +1,6 @@
-"""Batched boxes (nmhd_batch_*): many periodic boxes of uneven sizes in
-per-tile-shape or persistent launches over a work table, per-box dt,
-HIP-graph replay and the autotuner.
+"""Batched boxes (nmhd_batch_*): many boxes of uneven sizes (periodic or
+shearing) in per-tile-shape or persistent launches over a work table,
+per-box dt and ghost fills, HIP-graph replay and the autotuner.
 
 The batched kernels run the single-box tile functions, and a cell's
 arithmetic does not depend on the tiling, so the invariant checked here is
@@ -78,6 +78,54 @@ def test_batch_eager_equals_single_boxes(gpu_ready):
     # the boxes took different dt
     dts = {B.status(i)[2] for i in range(B.nbox)}
     assert len(dts) > 1
+
+
+def shearing_mix():
+    """Shearing boxes of uneven sizes (one with vertical ghost planes too,
+    one with all three quirk corrections) mixed with periodic boxes."""
+    out = []
+    for k, shape in enumerate([(16, 16, 8), (24, 16, 16), (32, 24, 8)]):
+        cfg = P.config_c2(shape)
+        if k == 2:
+            cfg = dict(cfg, fixes=capi.FIX_ALL)
+        out.append((cfg, P.shearing_box_mri(cfg, seed=40 + k)))
+    cfg = P.make_config((16, 24, 8), margin=(0, 4, 2), cfl=0.4, source=1, shear_y=1)
+    out.append((cfg, P.shearing_box_mri(cfg, seed=44)))
+    out += small_boxes(seed=31)[:3]
+    return out
+
+
+def test_batch_shearing_boxes_equal_single_boxes(gpu_ready):
+    """Shearing boxes in a batch (per-box shear remap of the radial ghost
+    rows at each box's own time, periodic vertical ghost planes, the ghost
+    cells' CFL added to each box's minimum), mixed with periodic boxes:
+    every box equals its single-box tiled context bit for bit, eagerly and
+    replayed from the graph, and across a goal that some boxes reach."""
+    announce("batch of shearing and periodic boxes vs single contexts")
+    boxes = shearing_mix()
+    ref = single_runs(boxes, 1e30, 6)
+    B = make_batch(boxes)
+    k, _ = B.run(1e30, 3, use_graph=False)
+    assert k == 3
+    k, _ = B.run(1e30, 3, use_graph=True)
+    assert k == 3
+    assert_batch_equals(B, ref)
+    # a goal ahead of every box's time: each box clips its own last step
+    # and stops there
+    goal = max(t for _, (t, _), _ in ref) + 0.02
+    ref2 = []
+    for (cfg, W), (Wr, (t, gen), _) in zip(boxes, ref):
+        g = capi.Context(cfg, path=capi.PATH_TILED)
+        g.set_state(Wr)
+        g.set_time(t, gen)
+        g.run(goal, 5)
+        ref2.append((g.get_state(), g.get_time(), 0))
+        g.close()
+    B.run(goal, 5)
+    assert_batch_equals(B, ref2)
+    reached = [B.status(i)[1] for i in range(B.nbox)]
+    assert any(reached), reached
+    B.close()
 
 
 def test_batch_with_cfl_speed_fix_equals_single_boxes(gpu_ready):
