@@ -82,7 +82,8 @@ def test_batch_eager_equals_single_boxes(gpu_ready):
 
 def shearing_mix():
     """Shearing boxes of uneven sizes (one with vertical ghost planes too,
-    one with all three quirk corrections) mixed with periodic boxes."""
+    one with all three quirk corrections, one with first-order
+    reconstruction) mixed with periodic boxes (one first-order)."""
     out = []
     for k, shape in enumerate([(16, 16, 8), (24, 16, 16), (32, 24, 8)]):
         cfg = P.config_c2(shape)
@@ -91,8 +92,12 @@ def shearing_mix():
         out.append((cfg, P.shearing_box_mri(cfg, seed=40 + k)))
     cfg = P.make_config((16, 24, 8), margin=(0, 4, 2), cfl=0.4, source=1, shear_y=1)
     out.append((cfg, P.shearing_box_mri(cfg, seed=44)))
-    out += small_boxes(seed=31)[:3]
-    return out
+    # first-order reconstruction in one shearing and one periodic box
+    cfg = dict(P.config_c2((16, 16, 16)), recon=1)
+    out.append((cfg, P.shearing_box_mri(cfg, seed=45)))
+    per = small_boxes(seed=31)[:3]
+    per[1] = (dict(per[1][0], recon=1), per[1][1])
+    return out + per
 
 
 def test_batch_shearing_boxes_equal_single_boxes(gpu_ready):
