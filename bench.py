@@ -114,6 +114,26 @@ def host_threads():
     return threads, {"nproc": nproc, "affinity_cpus": affinity, "OMP_NUM_THREADS": omp}
 
 
+def c1_leg():
+    """The 32^3 Orszag-Tang configuration (periodic, gamma 5/3, CFL 0.4) for
+    its full 100 steps from the initial condition on the default path: a
+    small box, where launch latency and the first step's full CFL pass
+    weigh (its parity with the oracle, bit for bit on the strict path, is a
+    GPU test)."""
+    from nmhd_amd import capi, problems as P
+    cfg = P.config_c1(32)
+    g = capi.Context(cfg)
+    g.set_state(P.orszag_tang(cfg))
+    g.synchronize()
+    t0 = time.perf_counter()
+    k, _ = g.run(1e30, 100)
+    el = time.perf_counter() - t0
+    t_end = g.get_time()[0]
+    g.close()
+    return {"value": P.interior_cells(cfg) * k / el, "unit": "zone-updates/s", "steps": k,
+            "ms_per_step": el / k * 1e3, "t_end": t_end}
+
+
 def cpu_baseline(cfg, W, steps):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle
@@ -132,6 +152,13 @@ def cpu_baseline(cfg, W, steps):
     one.step()
     el1 = time.perf_counter() - t1
     one.close()
+    # the 32^3 Orszag-Tang configuration for its full 100 steps, all threads
+    oc = P.config_c1(32)
+    ot = pyoracle.Oracle(oc, P.orszag_tang(oc), threads=threads)
+    t2 = time.perf_counter()
+    k2, _ = ot.run(1e30, 100)
+    el2 = time.perf_counter() - t2
+    ot.close()
     return {
         "value": P.interior_cells(cfg) * steps / el,
         "unit": "zone-updates/s",
@@ -143,6 +170,9 @@ def cpu_baseline(cfg, W, steps):
                   % ((steps,) + tuple(cfg["global_size"]) + (threads,)),
         "single_thread": {"value": P.interior_cells(c1) / el1, "unit": "zone-updates/s",
                           "sample": "1 step of the 128^3 box of the same setup, 1 thread"},
+        "c1_ot32": {"value": P.interior_cells(oc) * k2 / el2, "unit": "zone-updates/s",
+                    "seconds": el2,
+                    "sample": "32^3 Orszag-Tang, %d steps, %d threads" % (k2, threads)},
     }
 
 
@@ -498,6 +528,8 @@ def main():
     if large:
         out["stage_gbps"] = stage_rates(cfg, tim, kbytes, zones)
     ctx.close()
+    if not large:
+        out["c1_ot32"] = c1_leg()
     out["batched"] = None if a.no_batched else batched_leg(a, torch)
     out["adjoint"] = None if a.no_adjoint else adjoint_leg(a, torch)
     if not a.no_cpu:
